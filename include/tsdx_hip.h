@@ -1,0 +1,119 @@
+/* tsdx_hip.h -- C ABI of the MI355X (gfx950) batched small-matrix solver
+ * library (libtsdx_hip.so).
+ *
+ * Layout and semantics are the reference's (include/tinysolverdx/batched.hpp:1-6):
+ * A is (batch, n, n) row-major with matrices contiguous, 1 <= n <= 64; the
+ * Cholesky result is the lower factor with the upper triangle zeroed; LU is
+ * packed L\U with 0-based sequential pivots piv (batch, n) (LAPACK ipiv - 1);
+ * info (batch) is the LAPACK code (0 ok, k > 0: first failing pivot k), and
+ * may be NULL; right-hand sides B / solutions X are (batch, n, nrhs) row-major.
+ *
+ * Factors, pivots and info are bit-identical to the reference's Auto CPU
+ * backend (SIMD tile kernels for Cholesky and for LU up to n = 24, scalar LU
+ * kernel from n = 25).  Solutions agree to rounding (the backward substitution
+ * runs as a column sweep, a different summation order).
+ *
+ * Every function returns TSDX_HIP_OK, TSDX_HIP_EINVAL (bad size, batch, null
+ * pointer: the reference throws std::invalid_argument) or TSDX_HIP_ERUNTIME
+ * (no device, HIP failure: the reference throws std::runtime_error).  The
+ * message of the last failure on the calling thread is tsdx_hip_last_error().
+ *
+ * Two families:
+ *   tsdx_hip_<op>_<t>(..., stream)      device pointers, stream-ordered, no
+ *                                       synchronisation, no allocation;
+ *   tsdx_hip_<op>_batched_<t>(...)      host pointers, synchronous (stage to
+ *                                       the device, run, copy back), the shape
+ *                                       of the reference's CUDA backend.
+ * <t> is f32 or f64.  stream is a hipStream_t (NULL = the null stream).
+ */
+#ifndef TSDX_HIP_H
+#define TSDX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSDX_HIP_OK 0
+#define TSDX_HIP_EINVAL (-1)
+#define TSDX_HIP_ERUNTIME (-2)
+
+/* 1 when a HIP device is visible (replaces cuda::available(),
+ * include/tinysolverdx/cuda_backend.hpp:9 / src/cuda/cuda_backend.cu:150-153). */
+int tsdx_hip_available(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *tsdx_hip_last_error(void);
+
+/* Library version as major*10000 + minor*100 + patch. */
+int tsdx_hip_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Device-pointer entry points (stream-ordered).
+ * ------------------------------------------------------------------------ */
+
+/* potrf: A := lower Cholesky factor.  Replaces the device work of
+ * cuda::cholesky_batched (cuda_backend.hpp:10, cuda_backend.cu:155-170). */
+int tsdx_hip_potrf_f32(float *A, int n, int64_t batch, int *info, void *stream);
+int tsdx_hip_potrf_f64(double *A, int n, int64_t batch, int *info, void *stream);
+
+/* getrf: A := packed L\U, piv (batch, n) must not be NULL.  Replaces the device
+ * work of cuda::lu_batched (cuda_backend.hpp:11, cuda_backend.cu:172-188). */
+int tsdx_hip_getrf_f32(float *A, int n, int64_t batch, int *piv, int *info, void *stream);
+int tsdx_hip_getrf_f64(double *A, int n, int64_t batch, int *piv, int *info, void *stream);
+
+/* potrs: B := (L L^T)^-1 B.  Device form of cholesky_solve_batched
+ * (batched.hpp:248-257, which always runs on the CPU in the reference). */
+int tsdx_hip_potrs_f32(const float *L, int n, int64_t batch, float *B, int nrhs, void *stream);
+int tsdx_hip_potrs_f64(const double *L, int n, int64_t batch, double *B, int nrhs, void *stream);
+
+/* getrs: B := A^-1 B from (LU, piv).  Device form of lu_solve_batched
+ * (batched.hpp:259-268). */
+int tsdx_hip_getrs_f32(const float *LU, const int *piv, int n, int64_t batch, float *B, int nrhs, void *stream);
+int tsdx_hip_getrs_f64(const double *LU, const int *piv, int n, int64_t batch, double *B, int nrhs, void *stream);
+
+/* posv: A := L, B := X in one launch (the reference composes two passes,
+ * posv_batched, batched.hpp:271-275). */
+int tsdx_hip_posv_f32(float *A, int n, int64_t batch, float *B, int nrhs, int *info, void *stream);
+int tsdx_hip_posv_f64(double *A, int n, int64_t batch, double *B, int nrhs, int *info, void *stream);
+
+/* gesv: A := L\U, piv, B := X in one launch (gesv_batched, batched.hpp:276-280). */
+int tsdx_hip_gesv_f32(float *A, int n, int64_t batch, float *B, int nrhs, int *piv, int *info, void *stream);
+int tsdx_hip_gesv_f64(double *A, int n, int64_t batch, double *B, int nrhs, int *piv, int *info, void *stream);
+
+/* Solve-only posv / gesv: A and B are read, X is written, the factor is not
+ * stored (piv may be NULL).  No reference counterpart: it is the plain path
+ * with the algorithmic minimum of HBM traffic (read A and B, write X and info). */
+int tsdx_hip_posv_solve_f32(const float *A, int n, int64_t batch, const float *B, float *X, int nrhs, int *info,
+                            void *stream);
+int tsdx_hip_posv_solve_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs, int *info,
+                            void *stream);
+int tsdx_hip_gesv_solve_f32(const float *A, int n, int64_t batch, const float *B, float *X, int nrhs, int *piv,
+                            int *info, void *stream);
+int tsdx_hip_gesv_solve_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs, int *piv,
+                            int *info, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Host-pointer entry points (synchronous).  Shapes of the reference's CUDA
+ * backend (cuda_backend.hpp:10-11) plus the solve and driver calls of
+ * batched.hpp:248-280.
+ * ------------------------------------------------------------------------ */
+int tsdx_hip_cholesky_batched_f32(float *A, int n, int64_t batch, int *info);
+int tsdx_hip_cholesky_batched_f64(double *A, int n, int64_t batch, int *info);
+int tsdx_hip_lu_batched_f32(float *A, int n, int64_t batch, int *piv, int *info);
+int tsdx_hip_lu_batched_f64(double *A, int n, int64_t batch, int *piv, int *info);
+int tsdx_hip_cholesky_solve_batched_f32(const float *L, int n, int64_t batch, float *B, int nrhs);
+int tsdx_hip_cholesky_solve_batched_f64(const double *L, int n, int64_t batch, double *B, int nrhs);
+int tsdx_hip_lu_solve_batched_f32(const float *LU, const int *piv, int n, int64_t batch, float *B, int nrhs);
+int tsdx_hip_lu_solve_batched_f64(const double *LU, const int *piv, int n, int64_t batch, double *B, int nrhs);
+int tsdx_hip_posv_batched_f32(float *A, int n, int64_t batch, float *B, int nrhs, int *info);
+int tsdx_hip_posv_batched_f64(double *A, int n, int64_t batch, double *B, int nrhs, int *info);
+int tsdx_hip_gesv_batched_f32(float *A, int n, int64_t batch, float *B, int nrhs, int *piv, int *info);
+int tsdx_hip_gesv_batched_f64(double *A, int n, int64_t batch, double *B, int nrhs, int *piv, int *info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSDX_HIP_H */

@@ -1,0 +1,201 @@
+"""Parity of the HIP path with the reference, through the C ABI.
+
+Factors, pivots and info must be bit-identical to the golden fixtures written
+by the compiled reference and to the oracle on fresh inputs (any two NaNs
+count as equal).  Solutions agree to rounding: the backward substitution runs
+as a column sweep, so the bar is the reference's own accuracy, measured as a
+solve backward error below 8*n*u (fp32/fp64) plus closeness to the oracle's
+solution within c*kappa*n*u.
+"""
+import numpy as np
+import pytest
+
+import hipcalls as H
+from conftest import bits_equal, first_bit_mismatch, golden, golden_cases
+
+pytestmark = pytest.mark.gpu
+
+U = {np.dtype(np.float32): 2.0 ** -24, np.dtype(np.float64): 2.0 ** -53}
+
+
+def _spd(rng, b, n, dt, kappa=1e3):
+    from tinysolverdx import matrices
+    return matrices.spd(rng, b, n, kappa).astype(dt)
+
+
+def _solution_ok(A, X, B, Xref, dt):
+    """Backward error of each column below 8 n u and agreement with Xref."""
+    import tinysolverdx as tsdx
+    n = A.shape[-1]
+    u = U[np.dtype(dt)]
+    A64 = A.astype(np.float64)
+    for r in range(B.shape[-1]):
+        bw = tsdx.solve_backward_error(A64, X[..., r].astype(np.float64), B[..., r].astype(np.float64))
+        assert np.nanmax(bw) < 8 * n * u, (n, r, np.nanmax(bw))
+    kappa = np.linalg.cond(A64)
+    rel = np.linalg.norm(X - Xref, axis=1) / np.maximum(np.linalg.norm(Xref, axis=1), 1e-300)
+    assert np.all(rel <= 64 * n * u * kappa[:, None] + 1e-300), (n, rel.max())
+
+
+# -------------------------------------------------------------- golden fixtures
+@pytest.mark.parametrize("sfx", ["f32", "f64"])
+def test_potrf_golden_bit_exact(hip, sfx):
+    g = golden(f"potrf_{sfx}")
+    for c in golden_cases(g, "_A"):
+        print(f"[potrf golden] {sfx} {c}", flush=True)
+        L, info = H.cholesky(g[f"{c}_A"])
+        assert np.array_equal(info, g[f"{c}_info"]), (c, info, g[f"{c}_info"])
+        assert bits_equal(L, g[f"{c}_L"]), (c, first_bit_mismatch(L, g[f"{c}_L"]))
+
+
+@pytest.mark.parametrize("sfx", ["f32", "f64"])
+def test_getrf_golden_bit_exact(hip, sfx):
+    g = golden(f"getrf_{sfx}")
+    for c in golden_cases(g, "_A"):
+        print(f"[getrf golden] {sfx} {c}", flush=True)
+        LU, piv, info = H.lu(g[f"{c}_A"])
+        assert np.array_equal(piv, g[f"{c}_piv"]), (c, first_bit_mismatch(piv, g[f"{c}_piv"]))
+        assert np.array_equal(info, g[f"{c}_info"]), (c, info, g[f"{c}_info"])
+        assert bits_equal(LU, g[f"{c}_LU"]), (c, first_bit_mismatch(LU, g[f"{c}_LU"]))
+
+
+@pytest.mark.parametrize("sfx", ["f32", "f64"])
+def test_solves_golden(hip, sfx):
+    for kind in ("potrf", "getrf"):
+        g = golden(f"{kind}_{sfx}")
+        for c in golden_cases(g, "_A"):
+            A = g[f"{c}_A"]
+            if not np.isfinite(A).all() or A.shape[-1] < 1:
+                continue
+            if kind == "potrf":
+                if (g[f"{c}_info"] != 0).any():
+                    continue
+                X = H.cholesky_solve(g[f"{c}_L"], g[f"{c}_B"])
+            else:
+                if (g[f"{c}_info"] != 0).any() or c.startswith("subnormal"):
+                    continue
+                X = H.lu_solve(g[f"{c}_LU"], g[f"{c}_piv"], g[f"{c}_B"])
+            _solution_ok(A, X, g[f"{c}_B"], g[f"{c}_X"], A.dtype)
+
+
+# ---------------------------------------------------------- oracle, all sizes
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_factor_all_sizes_vs_oracle(hip, oracle, dt):
+    rng = np.random.default_rng(7)
+    for n in range(1, 65):
+        b = 67  # not a multiple of any group count: exercises the tail group
+        print(f"[factor sweep] {np.dtype(dt).name} n={n}", flush=True)
+        S = _spd(rng, b, n, dt)
+        S[3] = -np.eye(n, dtype=dt)
+        if n > 2:
+            S[5, n // 2, n // 2] = -1.0
+        L, info = H.cholesky(S)
+        Lo, infoo = oracle.cholesky(S)
+        assert np.array_equal(info, infoo), n
+        ok = infoo == 0
+        assert bits_equal(L[ok], Lo[ok]), (n, first_bit_mismatch(L[ok], Lo[ok]))
+        G = rng.standard_normal((b, n, n)).astype(dt)
+        G[2, :, n // 2] = 0
+        G[4] = 0
+        LU, piv, info = H.lu(G)
+        LUo, pivo, infoo = oracle.lu(G)
+        assert np.array_equal(piv, pivo), (n, first_bit_mismatch(piv, pivo))
+        assert np.array_equal(info, infoo), n
+        assert bits_equal(LU, LUo), (n, first_bit_mismatch(LU, LUo))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_failed_cholesky_matches_tile_semantics(hip, oracle, dt):
+    """After a failure the reference's Auto (SIMD tile) kernel substitutes 1 and
+    continues; the HIP kernel does the same, so even failed factors match."""
+    rng = np.random.default_rng(8)
+    for n in (5, 16, 33, 64):
+        S = _spd(rng, 9, n, dt)
+        S[:, n // 3, n // 3] = -2.0
+        L, info = H.cholesky(S)
+        Lo, infoo = oracle.cholesky(S)
+        assert np.array_equal(info, infoo) and (info > 0).all()
+        assert bits_equal(L, Lo), (n, first_bit_mismatch(L, Lo))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_fused_drivers_vs_oracle(hip, oracle, dt):
+    rng = np.random.default_rng(9)
+    for n in (1, 2, 3, 7, 8, 12, 16, 17, 24, 25, 32, 40, 63, 64):
+        for nrhs in (1, 3, 4, 9):
+            print(f"[fused] {np.dtype(dt).name} n={n} nrhs={nrhs}", flush=True)
+            S = _spd(rng, 23, n, dt)
+            G = rng.standard_normal((23, n, n)).astype(dt)
+            B = rng.standard_normal((23, n, nrhs)).astype(dt)
+            L, X, info = H.posv(S, B)
+            Lo, Xo, infoo = oracle.posv(S, B)
+            assert np.array_equal(info, infoo) and bits_equal(L, Lo), n
+            _solution_ok(S, X, B, Xo, dt)
+            LU, X, piv, info = H.gesv(G, B)
+            LUo, Xo, pivo, infoo = oracle.gesv(G, B)
+            assert np.array_equal(piv, pivo) and np.array_equal(info, infoo) and bits_equal(LU, LUo), n
+            _solution_ok(G, X, B, Xo, dt)
+            Xs = H.cholesky_solve(Lo, B)
+            _solution_ok(S, Xs, B, oracle.cholesky_solve(Lo, B), dt)
+            Xg = H.lu_solve(LUo, pivo, B)
+            _solution_ok(G, Xg, B, oracle.lu_solve(LUo, pivo, B), dt)
+
+
+def test_forward_pass_is_bit_exact(hip, oracle):
+    """With an identity upper factor only the forward pass does arithmetic, and
+    that pass keeps the reference's operation order: results are bit-equal."""
+    rng = np.random.default_rng(10)
+    for dt in (np.float32, np.float64):
+        for n in (3, 16, 31, 64):
+            LU = np.tril(rng.standard_normal((5, n, n)), -1).astype(dt) + np.eye(n, dtype=dt)
+            piv = np.tile(np.arange(n, dtype=np.int32), (5, 1))
+            B = rng.standard_normal((5, n, 4)).astype(dt)
+            assert bits_equal(H.lu_solve(LU, piv, B), oracle.lu_solve(LU, piv, B)), (dt, n)
+
+
+# ------------------------------------------------------------ device entry points
+def test_device_entry_points_and_unaligned_rows(hip, oracle):
+    import torch
+    from tinysolverdx import device
+    rng = np.random.default_rng(12)
+    for dt, tdt in ((np.float32, torch.float32), (np.float64, torch.float64)):
+        for n in (5, 8, 32, 33, 64):
+            S = _spd(rng, 50, n, dt)
+            G = rng.standard_normal((50, n, n)).astype(dt)
+            dS = torch.from_numpy(S).cuda()
+            info = torch.zeros(50, dtype=torch.int32, device="cuda")
+            device.potrf(dS, info)
+            Lo, io = oracle.cholesky(S)
+            torch.cuda.synchronize()
+            assert bits_equal(dS.cpu().numpy(), Lo) and np.array_equal(info.cpu().numpy(), io)
+            # rows starting at an odd element offset force the element-wise load path
+            buf = torch.zeros(G.size + 1, dtype=tdt, device="cuda")
+            dG = buf[1:].view(50, n, n)
+            dG.copy_(torch.from_numpy(G))
+            piv = torch.zeros((50, n), dtype=torch.int32, device="cuda")
+            device.getrf(dG, piv, info)
+            LUo, pivo, io = oracle.lu(G)
+            torch.cuda.synchronize()
+            assert bits_equal(dG.cpu().numpy(), LUo) and np.array_equal(piv.cpu().numpy(), pivo)
+            # solve-only drivers leave A and B untouched
+            B = rng.standard_normal((50, n, 2)).astype(dt)
+            dA, dB = torch.from_numpy(S).cuda(), torch.from_numpy(B).cuda()
+            dX = torch.empty_like(dB)
+            device.posv_solve(dA, dB, dX, info)
+            torch.cuda.synchronize()
+            assert bits_equal(dA.cpu().numpy(), S) and bits_equal(dB.cpu().numpy(), B)
+            _solution_ok(S, dX.cpu().numpy(), B, oracle.posv(S, B)[1], dt)
+            dA = torch.from_numpy(G).cuda()
+            device.gesv_solve(dA, dB, dX, None, info)
+            torch.cuda.synchronize()
+            assert bits_equal(dA.cpu().numpy(), G)
+            _solution_ok(G, dX.cpu().numpy(), B, oracle.gesv(G, B)[1], dt)
+
+
+def test_empty_and_null_info(hip):
+    import ctypes as C
+    from tinysolverdx import _lib
+    A = np.eye(4, dtype=np.float32)[None].copy()
+    assert _lib.lib().tsdx_hip_cholesky_batched_f32(A.ctypes.data_as(C.c_void_p), 4, 1, None) == 0
+    np.testing.assert_array_equal(A[0], np.eye(4))
+    assert _lib.lib().tsdx_hip_cholesky_batched_f32(A.ctypes.data_as(C.c_void_p), 4, 0, None) == 0

@@ -1,0 +1,177 @@
+"""Pins the oracle (oracle/tsdx_oracle.c) before anything is checked against it.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a separate multiply and add instead of one fused operation, a wrong tie
+rule, a transposed operand) fails one of them:
+  * golden fixtures written by the compiled reference (bit-exact);
+  * the compiled reference itself on fresh random inputs, when built here;
+  * closed forms (2x2 Cholesky, the row-swap example, zero columns, -I);
+  * library routines (numpy Cholesky, scipy lu_factor pivots, numpy solve);
+  * invariants (piv[k] in [k, n), backward error bounds 4nu / 8nu, upper
+    triangle zero, thread-count independence).
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import bits_equal, first_bit_mismatch, golden, golden_cases
+
+
+# --------------------------------------------------------------- golden fixtures
+@pytest.mark.parametrize("sfx", ["f32", "f64"])
+def test_potrf_matches_golden(oracle, sfx):
+    g = golden(f"potrf_{sfx}")
+    for c in golden_cases(g, "_A"):
+        A = g[f"{c}_A"]
+        L, info = oracle.cholesky(A, "auto")
+        assert np.array_equal(info, g[f"{c}_info"]), c
+        assert bits_equal(L, g[f"{c}_L"]), (c, first_bit_mismatch(L, g[f"{c}_L"]))
+        Ls, infos = oracle.cholesky(A, "scalar")
+        assert np.array_equal(infos, g[f"{c}_infoscalar"]), c
+        assert bits_equal(Ls, g[f"{c}_Lscalar"]), c
+        X = oracle.cholesky_solve(g[f"{c}_L"], g[f"{c}_B"])
+        assert bits_equal(X, g[f"{c}_X"]), c
+
+
+@pytest.mark.parametrize("sfx", ["f32", "f64"])
+def test_getrf_matches_golden(oracle, sfx):
+    g = golden(f"getrf_{sfx}")
+    for c in golden_cases(g, "_A"):
+        A = g[f"{c}_A"]
+        LU, piv, info = oracle.lu(A, "auto")
+        assert np.array_equal(piv, g[f"{c}_piv"]), c
+        assert np.array_equal(info, g[f"{c}_info"]), c
+        assert bits_equal(LU, g[f"{c}_LU"]), (c, first_bit_mismatch(LU, g[f"{c}_LU"]))
+        if f"{c}_LUscalar" in g.files:
+            LUs, pivs, infos = oracle.lu(A, "scalar")
+            assert np.array_equal(pivs, g[f"{c}_pivscalar"]) and np.array_equal(infos, g[f"{c}_infoscalar"]), c
+            assert bits_equal(LUs, g[f"{c}_LUscalar"]), c
+        X = oracle.lu_solve(g[f"{c}_LU"], g[f"{c}_piv"], g[f"{c}_B"])
+        assert bits_equal(X, g[f"{c}_X"]), c
+
+
+def test_metrics_match_golden(oracle):
+    g = golden("metrics_f64")
+    for c in golden_cases(g, "_A"):
+        A, S, b = g[f"{c}_A"], g[f"{c}_S"], g[f"{c}_b"]
+        np.testing.assert_array_equal(oracle.lu_backward_error(A, g[f"{c}_LU"], g[f"{c}_piv"]), g[f"{c}_lu_bwd"])
+        np.testing.assert_array_equal(oracle.chol_backward_error(S, g[f"{c}_L"]), g[f"{c}_chol_bwd"])
+        np.testing.assert_array_equal(oracle.solve_backward_error(A, g[f"{c}_xp"], b), g[f"{c}_solve_bwd"])
+        np.testing.assert_array_equal(oracle.solve_forward_error(A, g[f"{c}_xp"], b), g[f"{c}_solve_fwd"])
+        np.testing.assert_array_equal(oracle.reference_solve(A, b), g[f"{c}_x"])
+
+
+def test_precision_matches_golden_and_kats(oracle):
+    g = golden("precision")
+    got16 = np.array([oracle.round_fp16(v) for v in g["x"]], np.float32)
+    got8 = np.array([oracle.round_bf16(v) for v in g["x"]], np.float32)
+    assert bits_equal(got16, g["fp16"]) and bits_equal(got8, g["bf16"])
+    for p in ("fp32", "fp16", "bf16", "fp64"):
+        assert bits_equal(oracle.round_to(g["x64"], p), g[f"round_to_{p}"]), p
+    # known answers of tests/cpp/test_precision.cpp:10-42
+    r16, rb = oracle.round_fp16, oracle.round_bf16
+    assert r16(1.0 + 1 / 2048) == 1.0 and r16(1.0 + 3 / 2048) == 1.0 + 4 / 2048
+    assert r16(65504.0) == 65504.0 and np.isinf(r16(65520.0))
+    assert r16(6.0e-8) == np.float32(5.9604644775390625e-08) and r16(1e-9) == 0.0
+    assert r16(-0.1) == np.float32(-0.0999755859375) and np.isnan(r16(np.nan))
+    assert rb(1.0 + 1 / 256) == 1.0 and rb(1.0 + 3 / 256) == 1.0 + 4 / 256
+    assert rb(3.14159265) == np.float32(3.140625) and np.isinf(rb(np.finfo(np.float32).max))
+    assert [oracle.unit_roundoff(p) for p in ("fp32", "fp64", "fp16", "bf16")] == [2.0 ** -24, 2.0 ** -53, 2.0 ** -11,
+                                                                                    2.0 ** -8]
+
+
+# ------------------------------------------------ compiled reference, fresh inputs
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_oracle_equals_compiled_reference(oracle, ref_lib, dt):
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 3, 6, 8, 13, 16, 17, 24, 25, 31, 32, 40, 64):
+        b = 19
+        G = rng.standard_normal((b, n, n)).astype(dt)
+        M = rng.standard_normal((b, n, n))
+        S = (M @ M.transpose(0, 2, 1) + n * np.eye(n)).astype(dt)
+        for be in ("naive", "scalar", "simd", "auto"):
+            L1, i1 = oracle.cholesky(S, be)
+            L2, i2 = ref_lib.cholesky(S, be)
+            assert bits_equal(L1, L2) and np.array_equal(i1, i2), (n, be)
+            F1, p1, j1 = oracle.lu(G, be)
+            F2, p2, j2 = ref_lib.lu(G, be)
+            assert bits_equal(F1, F2) and np.array_equal(p1, p2) and np.array_equal(j1, j2), (n, be)
+        B = rng.standard_normal((b, n, 2)).astype(dt)
+        assert bits_equal(oracle.posv(S, B)[1], ref_lib.posv(S, B)[1]), n
+        assert bits_equal(oracle.gesv(G, B)[1], ref_lib.gesv(G, B)[1]), n
+
+
+# ------------------------------------------------------------------ closed forms
+def test_closed_forms(oracle):
+    L, info = oracle.cholesky(np.array([[[4.0, 2.0], [2.0, 3.0]]]))
+    assert info[0] == 0
+    np.testing.assert_array_equal(L[0], [[2.0, 0.0], [1.0, np.sqrt(2.0)]])
+    for be in ("naive", "scalar", "simd", "auto"):
+        LU, piv, info = oracle.lu(np.array([[[0.0, 1.0], [1.0, 0.0]]]), be)
+        assert list(piv[0]) == [1, 1] and info[0] == 0 and LU[0, 0, 0] == 1 and LU[0, 1, 1] == 1
+        _, info = oracle.cholesky(-np.eye(5)[None], be)
+        assert info[0] == 1
+    rng = np.random.default_rng(3)
+    for n in (4, 9, 30, 64):
+        for j in (0, n // 2, n - 1):
+            A = rng.standard_normal((1, n, n))
+            A[0, :, j] = 0.0
+            _, piv, info = oracle.lu(A)
+            assert info[0] == j + 1 and piv[0, j] == j, (n, j)
+
+
+# ------------------------------------------------------------- library routines
+@pytest.mark.parametrize("n", [2, 5, 16, 33, 64])
+def test_against_lapack(oracle, n):
+    rng = np.random.default_rng(n)
+    M = rng.standard_normal((7, n, n))
+    S = M @ M.transpose(0, 2, 1) + n * np.eye(n)
+    G = rng.standard_normal((7, n, n))
+    L, info = oracle.cholesky(S)
+    assert (info == 0).all()
+    np.testing.assert_allclose(L, np.linalg.cholesky(S), rtol=1e-12, atol=1e-12)
+    assert np.all(np.triu(L, 1) == 0)
+    LU, piv, info = oracle.lu(G)
+    for b in range(7):
+        ref_lu, ref_piv = sla.lu_factor(G[b])
+        np.testing.assert_array_equal(piv[b], ref_piv)
+        np.testing.assert_allclose(LU[b], ref_lu, rtol=1e-10, atol=1e-10)
+    B = rng.standard_normal((7, n, 3))
+    np.testing.assert_allclose(oracle.lu_solve(LU, piv, B), np.linalg.solve(G, B), rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(oracle.cholesky_solve(L, B), np.linalg.solve(S, B), rtol=1e-8, atol=1e-8)
+
+
+# -------------------------------------------------------------------- invariants
+@pytest.mark.parametrize("dt,u", [(np.float32, 2.0 ** -24), (np.float64, 2.0 ** -53)])
+def test_backward_error_bounds(oracle, dt, u):
+    rng = np.random.default_rng(5)
+    for n in (1, 3, 8, 17, 32, 64):
+        M = rng.standard_normal((11, n, n))
+        S = (M @ M.transpose(0, 2, 1) + n * np.eye(n)).astype(dt)
+        G = rng.standard_normal((11, n, n)).astype(dt)
+        L, _ = oracle.cholesky(S)
+        assert oracle.chol_backward_error(S.astype(np.float64), L).max() < 4 * u * n
+        LU, piv, info = oracle.lu(G)
+        assert (info == 0).all()
+        for k in range(n):
+            assert ((piv[:, k] >= k) & (piv[:, k] < n)).all()
+        assert oracle.lu_backward_error(G.astype(np.float64), LU, piv).max() < 8 * u * n
+
+
+def test_result_independent_of_thread_count(oracle):
+    rng = np.random.default_rng(4)
+    M = rng.standard_normal((3000, 12, 12))
+    S = M @ M.transpose(0, 2, 1) + 12 * np.eye(12)
+    oracle.set_num_threads(1)
+    a1 = oracle.cholesky(S)[0]
+    oracle.set_num_threads(4)
+    a4 = oracle.cholesky(S)[0]
+    oracle.set_num_threads(0)
+    assert bits_equal(a1, a4)
+
+
+def test_error_codes(oracle):
+    with pytest.raises(ValueError):
+        oracle.cholesky(np.zeros((1, 65, 65)))
+    with pytest.raises(RuntimeError):
+        oracle.cholesky(np.eye(3)[None], "cuda")

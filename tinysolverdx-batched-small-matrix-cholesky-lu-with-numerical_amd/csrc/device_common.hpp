@@ -1,0 +1,224 @@
+// Device-side building blocks shared by every kernel of the library.
+//
+// Work mapping used throughout ("row-per-lane groups"): a wave of 64 lanes is
+// split into G = 64/NB groups of NB lanes; group g owns one matrix and lane i of
+// the group holds row i of that matrix in registers (T r[NB], column index
+// static after unrolling).  NB is a compile-time bucket (4, 8, 16, 32, 64); the
+// runtime size n <= NB.  Lanes i >= n and groups past the end of the batch
+// still execute every step (so wave-wide shuffles and barriers stay uniform)
+// but never load or store.
+//
+// Cross-lane traffic inside a group:
+//   * broadcast of lane k (compile-time k): v_readlane for NB = 64,
+//     ds_swizzle (bit mode, and/or masks) for NB <= 32;
+//   * butterfly reductions: DPP quad_perm / row_half_mirror / row_mirror for
+//     the xor masks 1, 2, 7, 15, ds_swizzle for xor 16, ds_bpermute for xor 32;
+//   * row / column broadcasts of a whole vector: LDS, one buffer per wave.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+#include <utility>
+
+namespace tsdx_hip {
+
+constexpr int kWave = 64;
+
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+// Calls f(ic<B>), f(ic<B+1>), ..., f(ic<E-1>): indices stay constant
+// expressions, so register arrays indexed by them never touch scratch.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(ic<B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+// Calls f(ic<E-1>), ..., f(ic<B>).
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for_rev(F&& f) {
+  if constexpr (B < E) {
+    f(ic<E - 1>{});
+    static_for_rev<B, E - 1>(f);
+  }
+}
+
+// ------------------------------------------------------------- bit helpers
+__device__ __forceinline__ int lo32(double x) { return (int)(__builtin_bit_cast(unsigned long long, x) & 0xffffffffull); }
+__device__ __forceinline__ int hi32(double x) { return (int)(__builtin_bit_cast(unsigned long long, x) >> 32); }
+__device__ __forceinline__ double mk64(int lo, int hi) {
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// ---------------------------------------------------- broadcast of lane K
+template <int NB, int K>
+__device__ __forceinline__ int bcast_i32(int x) {
+  static_assert(K < NB, "lane outside group");
+  if constexpr (NB == 64) {
+    return __builtin_amdgcn_readlane(x, K);
+  } else {
+    // ds_swizzle bit mode: src = (lane & and_mask) | or_mask within 32 lanes
+    return __builtin_amdgcn_ds_swizzle(x, (0x1F & ~(NB - 1)) | (K << 5));
+  }
+}
+template <int NB, int K>
+__device__ __forceinline__ float bcast(float x) {
+  return __builtin_bit_cast(float, bcast_i32<NB, K>(__builtin_bit_cast(int, x)));
+}
+template <int NB, int K>
+__device__ __forceinline__ double bcast(double x) {
+  return mk64(bcast_i32<NB, K>(lo32(x)), bcast_i32<NB, K>(hi32(x)));
+}
+
+// Broadcast from a lane chosen at run time (any lane of the wave).
+__device__ __forceinline__ int bcast_dyn_i32(int x, int src_lane) {
+  return __builtin_amdgcn_ds_bpermute(src_lane << 2, x);
+}
+__device__ __forceinline__ float bcast_dyn(float x, int src_lane) {
+  return __builtin_bit_cast(float, bcast_dyn_i32(__builtin_bit_cast(int, x), src_lane));
+}
+__device__ __forceinline__ double bcast_dyn(double x, int src_lane) {
+  return mk64(bcast_dyn_i32(lo32(x), src_lane), bcast_dyn_i32(hi32(x), src_lane));
+}
+
+// ------------------------------------------------------ xor shuffles
+// M in {1, 2, 7, 15, 16, 32}: lane l receives lane l ^ M.
+template <int M>
+__device__ __forceinline__ int xshfl_i32(int x) {
+  if constexpr (M == 1) {
+    return __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (M == 2) {
+    return __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (M == 7) {
+    return __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  } else if constexpr (M == 15) {
+    return __builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false);  // row_mirror
+  } else if constexpr (M == 16) {
+    return __builtin_amdgcn_ds_swizzle(x, 0x401F);  // bit mode, xor_mask 16
+  } else {
+    static_assert(M == 32, "unsupported xor mask");
+    return __builtin_amdgcn_ds_bpermute((int)((__lane_id() ^ 32u) << 2), x);
+  }
+}
+template <int M>
+__device__ __forceinline__ float xshfl(float x) {
+  return __builtin_bit_cast(float, xshfl_i32<M>(__builtin_bit_cast(int, x)));
+}
+template <int M>
+__device__ __forceinline__ double xshfl(double x) {
+  return mk64(xshfl_i32<M>(lo32(x)), xshfl_i32<M>(hi32(x)));
+}
+template <int M>
+__device__ __forceinline__ int xshfl(int x) {
+  return xshfl_i32<M>(x);
+}
+
+// Butterfly all-reduce over the NB lanes of a group.  The xor masks
+// {1, 2, 7, 15, 16, 32} restricted to the group span every lane index of the
+// group, so after the last step every lane holds the reduction of the group.
+template <int NB, class V, class Op>
+__device__ __forceinline__ V group_allreduce(V x, Op op) {
+  static_assert(NB >= 4 && NB <= 64 && (NB & (NB - 1)) == 0, "group size");
+  x = op(x, xshfl<1>(x));
+  x = op(x, xshfl<2>(x));
+  if constexpr (NB >= 8) x = op(x, xshfl<7>(x));
+  if constexpr (NB >= 16) x = op(x, xshfl<15>(x));
+  if constexpr (NB >= 32) x = op(x, xshfl<16>(x));
+  if constexpr (NB >= 64) x = op(x, xshfl<32>(x));
+  return x;
+}
+
+// ------------------------------------------------------------ arithmetic
+// Fused multiply-add, one rounding (the reference's contracted `x -= y*z`).
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// IEEE square root (correctly rounded; the fp32 form relies on hipcc's default
+// -fhip-fp32-correctly-rounded-divide-sqrt).
+__device__ __forceinline__ float sqrtT(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double sqrtT(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float absT(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double absT(double x) { return __builtin_fabs(x); }
+template <class T>
+__device__ __forceinline__ T infT() {
+  return (T)__builtin_huge_val();
+}
+
+// Wave-local LDS hand-off: every kernel here runs one wave per workgroup, so
+// the workgroup barrier only orders the LDS traffic of that wave.
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// ------------------------------------------------------- row load / store
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int N = 4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int N = 2;
+};
+
+// Loads r[0..n-1] from p (contiguous).  vec: p is 16-byte aligned and n*sizeof(T)
+// is a multiple of 16, so 128-bit loads are used.  Other entries are zero.
+template <class T, int NB>
+__device__ __forceinline__ void load_row(T (&r)[NB], const T* __restrict__ p, int n, bool ok, bool vec) {
+  static_for<0, NB>([&](auto j) { r[j] = T(0); });
+  if (!ok) return;
+  constexpr int VE = Vec16<T>::N;
+  if constexpr (NB % VE == 0) {
+    if (vec) {
+      using V = typename Vec16<T>::type;
+      static_for<0, NB / VE>([&](auto q) {
+        if (q * VE < n) {
+          const V v = *reinterpret_cast<const V*>(p + q * VE);
+          if constexpr (VE == 4) {
+            r[q * 4 + 0] = v.x; r[q * 4 + 1] = v.y; r[q * 4 + 2] = v.z; r[q * 4 + 3] = v.w;
+          } else {
+            r[q * 2 + 0] = v.x; r[q * 2 + 1] = v.y;
+          }
+        }
+      });
+      return;
+    }
+  }
+  static_for<0, NB>([&](auto j) {
+    if (j < n) r[j] = p[j];
+  });
+}
+
+// Stores r[0..n-1] to p; with lower_only, entries j > i are written as zero.
+template <class T, int NB>
+__device__ __forceinline__ void store_row(const T (&r)[NB], T* __restrict__ p, int n, bool vec, bool lower_only, int i) {
+  constexpr int VE = Vec16<T>::N;
+  auto val = [&](int j, T x) { return (lower_only && j > i) ? T(0) : x; };
+  if constexpr (NB % VE == 0) {
+    if (vec) {
+      using V = typename Vec16<T>::type;
+      static_for<0, NB / VE>([&](auto q) {
+        if (q * VE < n) {
+          V v;
+          if constexpr (VE == 4) {
+            v.x = val(q * 4 + 0, r[q * 4 + 0]); v.y = val(q * 4 + 1, r[q * 4 + 1]);
+            v.z = val(q * 4 + 2, r[q * 4 + 2]); v.w = val(q * 4 + 3, r[q * 4 + 3]);
+          } else {
+            v.x = val(q * 2 + 0, r[q * 2 + 0]); v.y = val(q * 2 + 1, r[q * 2 + 1]);
+          }
+          *reinterpret_cast<V*>(p + q * VE) = v;
+        }
+      });
+      return;
+    }
+  }
+  static_for<0, NB>([&](auto j) {
+    if (j < n) p[j] = val(j, r[j]);
+  });
+}
+
+}  // namespace tsdx_hip

@@ -1,0 +1,49 @@
+// Umbrella header of the C++ host API (counterpart of the reference's
+// include/tinysolverdx/tinysolverdx.hpp): typed batched templates plus the
+// runtime-precision entry points, the long-double error metrics and thread
+// control for the host-side metric loops.
+#pragma once
+
+#include <cstdint>
+
+#include "batched.hpp"
+#include "precision.hpp"
+
+namespace tsdx {
+
+bool has_hip();   // a HIP device is visible to libtsdx_hip.so
+bool has_cuda();  // always false: this library has no CUDA backend
+
+int num_threads();
+void set_num_threads(int n);  // n <= 0 restores the default (TSDX_NUM_THREADS or hardware concurrency)
+
+// Runtime-precision API (reference runtime.cpp:95-150).  S is the storage type
+// (float or double).  FP32/FP64 run on the HIP backend, in place when the
+// storage already has that type, through a converting copy otherwise.  The
+// reference's emulated FP16/BF16 formats have no HIP implementation and raise
+// std::runtime_error (as the reference's CUDA backend does).
+template <class S>
+void cholesky(Precision p, S* A, int n, std::int64_t batch, int* info, Backend be = Backend::Auto);
+template <class S>
+void lu(Precision p, S* A, int n, std::int64_t batch, int* piv, int* info, Backend be = Backend::Auto);
+template <class S>
+void cholesky_solve(Precision p, const S* L, int n, std::int64_t batch, S* B, int nrhs);
+template <class S>
+void lu_solve(Precision p, const S* LU, const int* piv, int n, std::int64_t batch, S* B, int nrhs);
+template <class S>
+void posv(Precision p, S* A, int n, std::int64_t batch, S* B, int nrhs, int* info, Backend be = Backend::Auto);
+template <class S>
+void gesv(Precision p, S* A, int n, std::int64_t batch, S* B, int nrhs, int* piv, int* info,
+          Backend be = Backend::Auto);
+template <class S>
+void round_to(Precision p, S* x, std::int64_t count);
+
+// Long-double ground truth (inputs are doubles, one value per matrix),
+// definitions of the reference's runtime.cpp:162-260.
+void chol_backward_error(const double* A, const double* L, int n, std::int64_t batch, double* out);
+void lu_backward_error(const double* A, const double* LU, const int* piv, int n, std::int64_t batch, double* out);
+void solve_backward_error(const double* A, const double* x, const double* b, int n, std::int64_t batch, double* out);
+void solve_forward_error(const double* A, const double* x, const double* b, int n, std::int64_t batch, double* out);
+void reference_solve(const double* A, const double* b, int n, std::int64_t batch, double* x);
+
+}  // namespace tsdx

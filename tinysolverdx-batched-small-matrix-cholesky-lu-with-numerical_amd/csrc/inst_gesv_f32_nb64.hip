@@ -1,0 +1,3 @@
+// Instantiates the gesv kernels and launchers for float, matrix sizes n <= 64.
+#include "launch_impl.hpp"
+TSDX_INSTANTIATE_GESV(float, 64)

@@ -1,0 +1,3 @@
+// Instantiates the posv kernels and launchers for double, matrix sizes n <= 4.
+#include "launch_impl.hpp"
+TSDX_INSTANTIATE_POSV(double, 4)

@@ -1,0 +1,3 @@
+// Instantiates the solve kernels and launchers for float, matrix sizes n <= 32.
+#include "launch_impl.hpp"
+TSDX_INSTANTIATE_SOLVE(float, 32)

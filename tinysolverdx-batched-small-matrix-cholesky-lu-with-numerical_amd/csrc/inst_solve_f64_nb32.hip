@@ -1,0 +1,3 @@
+// Instantiates the solve kernels and launchers for double, matrix sizes n <= 32.
+#include "launch_impl.hpp"
+TSDX_INSTANTIATE_SOLVE(double, 32)

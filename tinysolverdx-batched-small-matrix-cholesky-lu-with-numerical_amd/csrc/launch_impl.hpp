@@ -1,0 +1,119 @@
+// Definitions of the launchers declared in launch.hpp.  Included only by the
+// inst_*.hip units, each of which instantiates one (family, type, NB) triple.
+#pragma once
+
+#include "launch.hpp"
+#include "plain_kernels.hpp"
+
+namespace tsdx_hip {
+
+template <int NB>
+inline dim3 grid_for(int64_t batch) {
+  constexpr int64_t G = kWave / NB;
+  return dim3((unsigned)((batch + G - 1) / G));
+}
+
+template <class T, int NB>
+hipError_t launch_potrf(T* A, int n, int64_t batch, int* info, int vec, hipStream_t s) {
+  hipLaunchKernelGGL((k_potrf<T, NB>), grid_for<NB>(batch), dim3(kWave), 0, s, A, n, batch, info, vec);
+  return hipGetLastError();
+}
+
+template <class T, int NB>
+hipError_t launch_getrf(T* A, int n, int64_t batch, int* piv, int* info, int vec, bool ztile, hipStream_t s) {
+  if constexpr (NB <= 16) {  // n <= 16: always the zero-pivot tile rule
+    hipLaunchKernelGGL((k_getrf<T, NB, true>), grid_for<NB>(batch), dim3(kWave), 0, s, A, n, batch, piv, info, vec);
+  } else if constexpr (NB == 64) {  // n >= 33: always the skip rule
+    hipLaunchKernelGGL((k_getrf<T, NB, false>), grid_for<NB>(batch), dim3(kWave), 0, s, A, n, batch, piv, info, vec);
+  } else {
+    if (ztile)
+      hipLaunchKernelGGL((k_getrf<T, NB, true>), grid_for<NB>(batch), dim3(kWave), 0, s, A, n, batch, piv, info, vec);
+    else
+      hipLaunchKernelGGL((k_getrf<T, NB, false>), grid_for<NB>(batch), dim3(kWave), 0, s, A, n, batch, piv, info, vec);
+  }
+  return hipGetLastError();
+}
+
+template <class T, int NB>
+hipError_t launch_potrs(const T* L, int n, int64_t batch, T* B, int nrhs, int vec, hipStream_t s) {
+  if (nrhs == 1)
+    hipLaunchKernelGGL((k_potrs<T, NB, 1>), grid_for<NB>(batch), dim3(kWave), 0, s, L, n, batch, B, nrhs, vec);
+  else
+    hipLaunchKernelGGL((k_potrs<T, NB, 4>), grid_for<NB>(batch), dim3(kWave), 0, s, L, n, batch, B, nrhs, vec);
+  return hipGetLastError();
+}
+
+template <class T, int NB>
+hipError_t launch_getrs(const T* LU, const int* piv, int n, int64_t batch, T* B, int nrhs, int vec, hipStream_t s) {
+  if (nrhs == 1)
+    hipLaunchKernelGGL((k_getrs<T, NB, 1>), grid_for<NB>(batch), dim3(kWave), 0, s, LU, piv, n, batch, B, nrhs, vec);
+  else
+    hipLaunchKernelGGL((k_getrs<T, NB, 4>), grid_for<NB>(batch), dim3(kWave), 0, s, LU, piv, n, batch, B, nrhs, vec);
+  return hipGetLastError();
+}
+
+template <class T, int NB>
+hipError_t launch_posv(const T* A, T* Lout, int n, int64_t batch, const T* B, T* X, int nrhs, int* info, int vec,
+                       hipStream_t s) {
+  if (nrhs <= 1)
+    hipLaunchKernelGGL((k_posv<T, NB, 1>), grid_for<NB>(batch), dim3(kWave), 0, s, A, Lout, n, batch, B, X, nrhs, info,
+                       vec);
+  else
+    hipLaunchKernelGGL((k_posv<T, NB, 4>), grid_for<NB>(batch), dim3(kWave), 0, s, A, Lout, n, batch, B, X, nrhs, info,
+                       vec);
+  return hipGetLastError();
+}
+
+template <class T, int NB, int NR>
+static void gesv_go(const T* A, T* LUout, int n, int64_t batch, const T* B, T* X, int nrhs, int* piv, int* info,
+                    int vec, bool ztile, hipStream_t s) {
+  if constexpr (NB <= 16) {
+    hipLaunchKernelGGL((k_gesv<T, NB, NR, true>), grid_for<NB>(batch), dim3(kWave), 0, s, A, LUout, n, batch, B, X,
+                       nrhs, piv, info, vec);
+  } else if constexpr (NB == 64) {
+    hipLaunchKernelGGL((k_gesv<T, NB, NR, false>), grid_for<NB>(batch), dim3(kWave), 0, s, A, LUout, n, batch, B, X,
+                       nrhs, piv, info, vec);
+  } else {
+    if (ztile)
+      hipLaunchKernelGGL((k_gesv<T, NB, NR, true>), grid_for<NB>(batch), dim3(kWave), 0, s, A, LUout, n, batch, B, X,
+                         nrhs, piv, info, vec);
+    else
+      hipLaunchKernelGGL((k_gesv<T, NB, NR, false>), grid_for<NB>(batch), dim3(kWave), 0, s, A, LUout, n, batch, B, X,
+                         nrhs, piv, info, vec);
+  }
+}
+
+template <class T, int NB>
+hipError_t launch_gesv(const T* A, T* LUout, int n, int64_t batch, const T* B, T* X, int nrhs, int* piv, int* info,
+                       int vec, bool ztile, hipStream_t s) {
+  if (nrhs <= 1)
+    gesv_go<T, NB, 1>(A, LUout, n, batch, B, X, nrhs, piv, info, vec, ztile, s);
+  else
+    gesv_go<T, NB, 4>(A, LUout, n, batch, B, X, nrhs, piv, info, vec, ztile, s);
+  return hipGetLastError();
+}
+
+}  // namespace tsdx_hip
+
+// One macro per kernel family so that each (type, NB, family) triple compiles
+// in its own translation unit.
+#define TSDX_INSTANTIATE_FACTOR(T, NB)                                                                   \
+  namespace tsdx_hip {                                                                                   \
+  template hipError_t launch_potrf<T, NB>(T*, int, int64_t, int*, int, hipStream_t);                     \
+  template hipError_t launch_getrf<T, NB>(T*, int, int64_t, int*, int*, int, bool, hipStream_t);         \
+  }
+#define TSDX_INSTANTIATE_SOLVE(T, NB)                                                                    \
+  namespace tsdx_hip {                                                                                   \
+  template hipError_t launch_potrs<T, NB>(const T*, int, int64_t, T*, int, int, hipStream_t);            \
+  template hipError_t launch_getrs<T, NB>(const T*, const int*, int, int64_t, T*, int, int, hipStream_t);\
+  }
+#define TSDX_INSTANTIATE_POSV(T, NB)                                                                     \
+  namespace tsdx_hip {                                                                                   \
+  template hipError_t launch_posv<T, NB>(const T*, T*, int, int64_t, const T*, T*, int, int*, int,       \
+                                         hipStream_t);                                                   \
+  }
+#define TSDX_INSTANTIATE_GESV(T, NB)                                                                     \
+  namespace tsdx_hip {                                                                                   \
+  template hipError_t launch_gesv<T, NB>(const T*, T*, int, int64_t, const T*, T*, int, int*, int*, int, \
+                                         bool, hipStream_t);                                             \
+  }

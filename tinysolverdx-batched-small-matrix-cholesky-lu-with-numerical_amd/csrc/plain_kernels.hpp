@@ -1,0 +1,366 @@
+// Plain fp32 / fp64 kernels: potrf, getrf, potrs, getrs and the fused
+// posv / gesv drivers, for one bucket NB of matrix sizes (n <= NB).
+//
+// Arithmetic contract (what makes factors, pivots and info bit-identical to the
+// reference CPU kernels, include/tinysolverdx/kernels.hpp:18-180):
+//   * every trailing update is one fma(-x, y, acc), applied to each element in
+//     ascending step order k, exactly the chain of the reference's contracted
+//     `acc -= x*y` (right-looking order visits k in the same order as the
+//     reference's left-looking dot products);
+//   * multipliers and off-diagonal Cholesky entries come from true IEEE
+//     division, diagonals from IEEE sqrt;
+//   * Cholesky failure (!(s > 0)) records the first failing index, substitutes
+//     1 and continues (the reference's SIMD tile semantics, kernels.hpp:70-76,
+//     which its Auto backend uses for fp32/fp64);
+//   * LU pivot = first position holding the largest |a_ik| among non-NaN values,
+//     a NaN diagonal keeps p = k (kernels.hpp:95-100); a zero pivot records info
+//     and either divides by 1 (ZTILE, the tile kernel used by the reference's
+//     Auto up to n = 24, kernels.hpp:165-170) or skips the column (the scalar
+//     kernel Auto uses from n = 25, kernels.hpp:109).
+// Columns j >= n are updated like the others (no per-column guard): they only
+// ever feed columns >= n, which are never stored, and lanes i >= n hold zero
+// rows, so the padded work cannot reach a stored value.
+// The triangular solves keep the forward passes in the reference's order and
+// run the backward passes right-looking (column sweeps), which is a different
+// summation order: solutions agree to rounding, not bit for bit.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace tsdx_hip {
+
+// ------------------------------------------------------------ Cholesky
+// r: row i of A on entry, row i of L on exit (entries j > i are garbage).
+// col: per-wave LDS, 2*64 elements.  Returns the group's LAPACK info.
+template <class T, int NB>
+__device__ __forceinline__ int chol_regs(T (&r)[NB], int n, int i, int g, T* col) {
+  int info = 0;
+  static_for<0, NB>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k < n) {
+      const T akk = bcast<NB, k>(r[k]);
+      const bool bad = !(akk > T(0));
+      info = (bad && info == 0) ? k + 1 : info;
+      const T d = sqrtT(bad ? T(1) : akk);
+      const T l = r[k] / d;
+      r[k] = (i == k) ? d : ((i > k) ? l : r[k]);
+      if constexpr (k + 1 < NB) {
+        T* cb = col + (k & 1) * kWave + g * NB;
+        cb[i] = r[k];
+        wave_sync();
+        static_for<k + 1, NB>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          r[j] = fmaT(-r[k], cb[j], r[j]);
+        });
+      }
+    }
+  });
+  return info;
+}
+
+// ------------------------------------------------------------------- LU
+// Rows never move between lanes: lane i keeps original row i and tracks its
+// current position `pos`.  On exit row `pos` of the packed L\U factor is in r,
+// lane k holds piv[k] in mypiv.  rowb: per-wave LDS, 2*64 elements.
+template <class T, int NB, bool ZTILE>
+__device__ __forceinline__ void lu_regs(T (&r)[NB], int n, int i, int g, T* rowb, int& pos, int& mypiv, int& info) {
+  pos = i;
+  mypiv = 0;
+  info = 0;
+  const bool act = i < n;
+  static_for<0, NB>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k < n) {
+      const bool cand = act && pos >= k;
+      const T v = absT(r[k]);
+      T key = (v != v) ? (pos == k ? infT<T>() : T(-1)) : v;
+      key = cand ? key : T(-2);
+      const T mx = group_allreduce<NB>(key, [](T a, T b) { return a > b ? a : b; });
+      const int cp = (cand && key == mx) ? pos : 0x7fffffff;
+      const int p = group_allreduce<NB>(cp, [](int a, int b) { return a < b ? a : b; });
+      if (i == k) mypiv = p;
+      const bool isp = act && pos == p;
+      pos = (pos == k) ? p : (isp ? k : pos);
+      T* rb = rowb + (k & 1) * kWave + g * NB;
+      if (isp) {
+        static_for<k, NB>([&](auto jc) { rb[jc] = r[jc]; });
+      }
+      wave_sync();
+      const T pk = rb[k];
+      const bool zero = (pk == T(0));
+      info = (zero && info == 0) ? k + 1 : info;
+      if (act && pos > k && (ZTILE || !zero)) {
+        const T f = r[k] / (zero ? T(1) : pk);
+        r[k] = f;
+        static_for<k + 1, NB>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          r[j] = fmaT(-f, rb[j], r[j]);
+        });
+      }
+    }
+  });
+}
+
+// ---------------------------------------------------------------- solves
+// Forward pass with L held row-per-lane (lane == row): y_i = (b_i - sum_{k<i}
+// L_ik y_k) / L_ii, updates applied in ascending k as the reference does.
+template <class T, int NB, int NR, bool UNIT>
+__device__ __forceinline__ void forward_rows(const T (&r)[NB], T (&b)[NR], int n, int i) {
+  static_for<0, NB>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k < n) {
+      static_for<0, NR>([&](auto c) {
+        const T yk = UNIT ? bcast<NB, k>(b[c]) : bcast<NB, k>(b[c] / r[k]);
+        b[c] = (i == k) ? yk : (i > k ? fmaT(-r[k], yk, b[c]) : b[c]);
+      });
+    }
+  });
+}
+
+// Backward pass with an upper factor held row-per-lane (U_ik = r[k] in lane i):
+// column sweep k = n-1..0, x_k = b_k / U_kk, then b_i -= U_ik x_k for i < k.
+template <class T, int NB, int NR>
+__device__ __forceinline__ void backward_rows_upper(const T (&r)[NB], T (&b)[NR], int n, int i) {
+  static_for_rev<0, NB>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k < n) {
+      static_for<0, NR>([&](auto c) {
+        const T xk = bcast<NB, k>(b[c] / r[k]);
+        b[c] = (i == k) ? xk : (i < k ? fmaT(-r[k], xk, b[c]) : b[c]);
+      });
+    }
+  });
+}
+
+// Backward pass L^T x = y with L held row-per-lane: row k of L (lane k) feeds
+// rows i < k, so blocks of CH rows of L are transposed through LDS.
+// tb: per-wave LDS, 64*CH elements.
+template <class T, int NB, int NR>
+__device__ __forceinline__ void backward_rows_lt(const T (&r)[NB], T (&b)[NR], int n, int i, int g, T* tb) {
+  constexpr int CH = NB < 8 ? NB : 8;
+  T* tg = tb + g * (CH * NB);
+  static_for_rev<0, NB / CH>([&](auto qc) {
+    constexpr int c0 = decltype(qc)::value * CH;
+    if (c0 < n) {
+      if (i >= c0 && i < c0 + CH) {
+        T* dst = tg + (i - c0) * NB;
+        static_for<0, c0 + CH>([&](auto jc) { dst[jc] = r[jc]; });
+      }
+      wave_sync();
+      static_for_rev<c0, c0 + CH>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if (k < n) {
+          const T lki = tg[(k - c0) * NB + i];
+          static_for<0, NR>([&](auto c) {
+            const T xk = bcast<NB, k>(b[c] / r[k]);
+            b[c] = (i == k) ? xk : (i < k ? fmaT(-lki, xk, b[c]) : b[c]);
+          });
+        }
+      });
+      wave_sync();
+    }
+  });
+}
+
+// Forward / backward passes for LU factors whose rows sit at dynamic positions
+// (lane i holds factor row pos, and right-hand-side row i, which is row pos of
+// P*B).  yb: per-wave LDS, 2*64 elements.
+template <class T, int NB, int NR>
+__device__ __forceinline__ void lu_solve_pos(const T (&r)[NB], T (&b)[NR], int n, int pos, int g, T* yb) {
+  static_for<0, NB>([&](auto tc) {
+    constexpr int t = decltype(tc)::value;
+    if (t < n) {
+      T* yg = yb + (t & 1) * kWave + g * NB;
+      if (pos == t) static_for<0, NR>([&](auto c) { yg[c] = b[c]; });
+      wave_sync();
+      static_for<0, NR>([&](auto c) {
+        const T y = yg[c];
+        if (pos > t) b[c] = fmaT(-r[t], y, b[c]);
+      });
+    }
+  });
+  static_for_rev<0, NB>([&](auto tc) {
+    constexpr int t = decltype(tc)::value;
+    if (t < n) {
+      T* yg = yb + (t & 1) * kWave + g * NB;
+      if (pos == t) static_for<0, NR>([&](auto c) {
+        b[c] = b[c] / r[t];
+        yg[c] = b[c];
+      });
+      wave_sync();
+      static_for<0, NR>([&](auto c) {
+        const T x = yg[c];
+        if (pos < t) b[c] = fmaT(-r[t], x, b[c]);
+      });
+    }
+  });
+}
+
+template <class T, int NR>
+__device__ __forceinline__ void load_rhs(T (&b)[NR], const T* __restrict__ p, int cnt, bool ok) {
+  static_for<0, NR>([&](auto c) { b[c] = (ok && c < cnt) ? p[c] : T(0); });
+}
+template <class T, int NR>
+__device__ __forceinline__ void store_rhs(const T (&b)[NR], T* __restrict__ p, int cnt, bool ok) {
+  if (ok) static_for<0, NR>([&](auto c) {
+      if (c < cnt) p[c] = b[c];
+    });
+}
+
+// ---------------------------------------------------------------- kernels
+struct Tile {
+  int lane, g, i;
+  int64_t m;
+  bool live;
+};
+template <int NB>
+__device__ __forceinline__ Tile tile_of(int64_t batch) {
+  Tile t;
+  t.lane = threadIdx.x;
+  t.g = t.lane / NB;
+  t.i = t.lane % NB;
+  t.m = (int64_t)blockIdx.x * (kWave / NB) + t.g;
+  t.live = t.m < batch;
+  return t;
+}
+
+template <class T, int NB>
+__global__ __launch_bounds__(kWave) void k_potrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ info, int vec) {
+  __shared__ T col[2 * kWave];
+  const Tile t = tile_of<NB>(batch);
+  const int64_t nn = (int64_t)n * n;
+  T* row = A + t.m * nn + (int64_t)t.i * n;
+  const bool io = t.live && t.i < n;
+  T r[NB];
+  load_row<T, NB>(r, row, n, io, vec != 0);
+  const int inf = chol_regs<T, NB>(r, n, t.i, t.g, col);
+  if (io) store_row<T, NB>(r, row, n, vec != 0, true, t.i);
+  if (t.live && t.i == 0 && info) info[t.m] = inf;
+}
+
+template <class T, int NB, bool ZTILE>
+__global__ __launch_bounds__(kWave) void k_getrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ piv,
+                                                 int* __restrict__ info, int vec) {
+  __shared__ T rowb[2 * kWave];
+  const Tile t = tile_of<NB>(batch);
+  const int64_t nn = (int64_t)n * n;
+  T* base = A + t.m * nn;
+  const bool io = t.live && t.i < n;
+  T r[NB];
+  load_row<T, NB>(r, base + (int64_t)t.i * n, n, io, vec != 0);
+  int pos, mypiv, inf;
+  lu_regs<T, NB, ZTILE>(r, n, t.i, t.g, rowb, pos, mypiv, inf);
+  if (io) {
+    store_row<T, NB>(r, base + (int64_t)pos * n, n, vec != 0, false, 0);
+    piv[t.m * n + t.i] = mypiv;
+  }
+  if (t.live && t.i == 0 && info) info[t.m] = inf;
+}
+
+// potrs: L from memory.  Forward pass on rows of L, backward pass on columns of
+// L (lane i loads column i, which is row i of L^T; loads are coalesced across
+// lanes because lane i reads element i of each row).
+template <class T, int NB, int NR>
+__global__ __launch_bounds__(kWave) void k_potrs(const T* __restrict__ L, int n, int64_t batch, T* __restrict__ B,
+                                                 int nrhs, int vec) {
+  const Tile t = tile_of<NB>(batch);
+  const int64_t nn = (int64_t)n * n;
+  const T* base = L + t.m * nn;
+  const bool io = t.live && t.i < n;
+  T r[NB];
+  for (int c0 = 0; c0 < nrhs; c0 += NR) {
+    T b[NR];
+    T* brow = B + (t.m * n + t.i) * (int64_t)nrhs + c0;
+    load_rhs<T, NR>(b, brow, nrhs - c0, io);
+    load_row<T, NB>(r, base + (int64_t)t.i * n, n, io, vec != 0);
+    forward_rows<T, NB, NR, false>(r, b, n, t.i);
+    static_for<0, NB>([&](auto k) { r[k] = (io && k < n) ? base[(int64_t)k * n + t.i] : T(0); });
+    backward_rows_upper<T, NB, NR>(r, b, n, t.i);
+    store_rhs<T, NR>(b, brow, nrhs - c0, io);
+  }
+}
+
+// getrs: LU from memory (row t at lane t); right-hand-side row t is row
+// perm[t] of B where perm applies the sequential swaps piv[0..n-1].
+template <class T, int NB, int NR>
+__global__ __launch_bounds__(kWave) void k_getrs(const T* __restrict__ LU, const int* __restrict__ piv, int n,
+                                                 int64_t batch, T* __restrict__ B, int nrhs, int vec) {
+  __shared__ int pv[kWave];
+  const Tile t = tile_of<NB>(batch);
+  const int64_t nn = (int64_t)n * n;
+  const bool io = t.live && t.i < n;
+  pv[t.lane] = io ? piv[t.m * n + t.i] : 0;
+  wave_sync();
+  // source row of P*B for position i: undo the swaps from the last one back
+  int src = t.i;
+  for (int k = n - 1; k >= 0; --k) {
+    const int p = pv[t.g * NB + k];
+    src = (src == k) ? p : (src == p ? k : src);
+  }
+  T r[NB];
+  load_row<T, NB>(r, LU + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
+  for (int c0 = 0; c0 < nrhs; c0 += NR) {
+    T b[NR];
+    load_rhs<T, NR>(b, B + (t.m * n + src) * (int64_t)nrhs + c0, nrhs - c0, io);
+    wave_sync();  // every lane has read its source row before any row is rewritten
+    forward_rows<T, NB, NR, true>(r, b, n, t.i);
+    backward_rows_upper<T, NB, NR>(r, b, n, t.i);
+    store_rhs<T, NR>(b, B + (t.m * n + t.i) * (int64_t)nrhs + c0, nrhs - c0, io);
+  }
+}
+
+// posv: potrf + potrs with the factor kept in registers.  Lout == nullptr
+// leaves A untouched (solve-only use); otherwise the factor is written there.
+template <class T, int NB, int NR>
+__global__ __launch_bounds__(kWave) void k_posv(const T* __restrict__ A, T* __restrict__ Lout, int n, int64_t batch,
+                                                const T* __restrict__ Bin, T* __restrict__ X, int nrhs,
+                                                int* __restrict__ info, int vec) {
+  constexpr int CH = NB < 8 ? NB : 8;
+  __shared__ T col[2 * kWave];
+  __shared__ T tb[kWave * CH];
+  const Tile t = tile_of<NB>(batch);
+  const int64_t nn = (int64_t)n * n;
+  const bool io = t.live && t.i < n;
+  T r[NB];
+  load_row<T, NB>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
+  const int inf = chol_regs<T, NB>(r, n, t.i, t.g, col);
+  if (Lout && io) store_row<T, NB>(r, Lout + t.m * nn + (int64_t)t.i * n, n, vec != 0, true, t.i);
+  if (t.live && t.i == 0 && info) info[t.m] = inf;
+  for (int c0 = 0; c0 < nrhs; c0 += NR) {
+    T b[NR];
+    const int64_t off = (t.m * n + t.i) * (int64_t)nrhs + c0;
+    load_rhs<T, NR>(b, Bin + off, nrhs - c0, io);
+    forward_rows<T, NB, NR, false>(r, b, n, t.i);
+    backward_rows_lt<T, NB, NR>(r, b, n, t.i, t.g, tb);
+    store_rhs<T, NR>(b, X + off, nrhs - c0, io);
+  }
+}
+
+// gesv: getrf + getrs with the factor kept in registers.
+template <class T, int NB, int NR, bool ZTILE>
+__global__ __launch_bounds__(kWave) void k_gesv(const T* __restrict__ A, T* __restrict__ LUout, int n, int64_t batch,
+                                                const T* __restrict__ Bin, T* __restrict__ X, int nrhs,
+                                                int* __restrict__ piv, int* __restrict__ info, int vec) {
+  __shared__ T rowb[2 * kWave];
+  const Tile t = tile_of<NB>(batch);
+  const int64_t nn = (int64_t)n * n;
+  const bool io = t.live && t.i < n;
+  T r[NB];
+  load_row<T, NB>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
+  int pos, mypiv, inf;
+  lu_regs<T, NB, ZTILE>(r, n, t.i, t.g, rowb, pos, mypiv, inf);
+  if (io) {
+    if (LUout) store_row<T, NB>(r, LUout + t.m * nn + (int64_t)pos * n, n, vec != 0, false, 0);
+    if (piv) piv[t.m * n + t.i] = mypiv;
+  }
+  if (t.live && t.i == 0 && info) info[t.m] = inf;
+  for (int c0 = 0; c0 < nrhs; c0 += NR) {
+    T b[NR];
+    load_rhs<T, NR>(b, Bin + (t.m * n + t.i) * (int64_t)nrhs + c0, nrhs - c0, io);
+    wave_sync();
+    lu_solve_pos<T, NB, NR>(r, b, n, pos, t.g, rowb);
+    store_rhs<T, NR>(b, X + (t.m * n + pos) * (int64_t)nrhs + c0, nrhs - c0, io);
+  }
+}
+
+}  // namespace tsdx_hip

@@ -1,0 +1,74 @@
+"""ctypes binding of libtsdx_hip.so, the C ABI declared in include/tsdx_hip.h.
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in this package.
+"""
+import ctypes as C
+import pathlib
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libtsdx_hip.so"
+
+OK, EINVAL, ERUNTIME = 0, -1, -2
+
+_vp, _i, _i64, _d = C.c_void_p, C.c_int, C.c_int64, C.c_double
+
+# name -> argument types (every function returns int unless listed in _RES)
+SIGNATURES = {
+    "tsdx_hip_available": [],
+    "tsdx_hip_last_error": [],
+    "tsdx_hip_version": [],
+}
+for _t in ("f32", "f64"):
+    SIGNATURES.update({
+        f"tsdx_hip_potrf_{_t}": [_vp, _i, _i64, _vp, _vp],
+        f"tsdx_hip_getrf_{_t}": [_vp, _i, _i64, _vp, _vp, _vp],
+        f"tsdx_hip_potrs_{_t}": [_vp, _i, _i64, _vp, _i, _vp],
+        f"tsdx_hip_getrs_{_t}": [_vp, _vp, _i, _i64, _vp, _i, _vp],
+        f"tsdx_hip_posv_{_t}": [_vp, _i, _i64, _vp, _i, _vp, _vp],
+        f"tsdx_hip_gesv_{_t}": [_vp, _i, _i64, _vp, _i, _vp, _vp, _vp],
+        f"tsdx_hip_posv_solve_{_t}": [_vp, _i, _i64, _vp, _vp, _i, _vp, _vp],
+        f"tsdx_hip_gesv_solve_{_t}": [_vp, _i, _i64, _vp, _vp, _i, _vp, _vp, _vp],
+        f"tsdx_hip_cholesky_batched_{_t}": [_vp, _i, _i64, _vp],
+        f"tsdx_hip_lu_batched_{_t}": [_vp, _i, _i64, _vp, _vp],
+        f"tsdx_hip_cholesky_solve_batched_{_t}": [_vp, _i, _i64, _vp, _i],
+        f"tsdx_hip_lu_solve_batched_{_t}": [_vp, _vp, _i, _i64, _vp, _i],
+        f"tsdx_hip_posv_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp],
+        f"tsdx_hip_gesv_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp, _vp],
+    })
+_RES = {"tsdx_hip_last_error": C.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (loaded once)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built: run build.py (tinysolverdx-hip has no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RES.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().tsdx_hip_last_error().decode()
+
+
+def check(rc, what=""):
+    """Raises the exception the reference's bindings raise for the status."""
+    if rc == OK:
+        return
+    msg = last_error() or what
+    if rc == EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args), name)
