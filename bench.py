@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the batched posv path on one MI355X.
+
+Workload (the headline metric): posv at n = 64 on 131072 SPD float64 matrices,
+A = Q diag(lambda) Q^T with Q Haar-orthogonal, lambda_0 = 1, lambda_63 = 1/kappa
+and the other eigenvalues log-uniform in between, kappa log-uniform on
+[1e1, 1e6] per matrix; B ~ N(0, 1), nrhs = 1 (--nrhs changes it).  Inputs are
+generated on the device and stay resident in HBM; one step = one call over
+the whole batch.  Every solution is judged by its float64 normwise backward
+error ||b - Ax|| / (||A||_F ||x|| + ||b||), which must be <= 1e-14.
+
+Printed on stdout: one JSON line with the metric, the roofline of the dominant
+kernel (HIP events on the launch stream) and the reference's own CPU posv
+(compiled from its sources, oracle/_ref) timed on this host's cores.  Progress
+goes to stderr.
+
+    python bench.py [--steps K] [--warmup W] [--nrhs R] [--batch N] [--extra]
+"""
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+PKG_DIR = ROOT / "tinysolverdx-batched-small-matrix-cholesky-lu-with-numerical_amd"
+sys.path.insert(0, str(PKG_DIR))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "posv solves/s n=64: routed mixed+IR vs plain FP64 GPU at bwd err≤1e-14; HBM GB/s"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_spd(n, batch, kmin, kmax, seed, chunk=8192):
+    """SPD float64 matrices on the device with prescribed per-matrix kappa."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A = torch.empty((batch, n, n), dtype=torch.float64, device="cuda")
+    kappa = 10.0 ** (torch.rand(batch, generator=g, device="cuda", dtype=torch.float64) *
+                     (torch.log10(torch.tensor(kmax)) - torch.log10(torch.tensor(kmin))).item() +
+                     torch.log10(torch.tensor(kmin)).item())
+    for s in range(0, batch, chunk):
+        e = min(batch, s + chunk)
+        m = e - s
+        z = torch.randn((m, n, n), generator=g, device="cuda", dtype=torch.float64)
+        q, r = torch.linalg.qr(z)
+        q = q * torch.sign(torch.diagonal(r, dim1=1, dim2=2)).unsqueeze(1)
+        t = torch.rand((m, n), generator=g, device="cuda", dtype=torch.float64)
+        t[:, 0] = 0.0
+        t[:, n - 1] = 1.0
+        lam = kappa[s:e, None] ** (-t)  # 1 and 1/kappa at the ends, log-uniform inside
+        a = torch.matmul(q * lam.unsqueeze(1), q.transpose(1, 2).contiguous())
+        A[s:e] = 0.5 * (a + a.transpose(1, 2))
+    return A, kappa
+
+
+def backward_error(A, X, B):
+    """float64 normwise backward error per matrix (max over right-hand sides)."""
+    import torch
+    R = B - torch.matmul(A, X)
+    num = torch.linalg.vector_norm(R, dim=1)
+    den = torch.linalg.matrix_norm(A, "fro")[:, None] * torch.linalg.vector_norm(X, dim=1) + \
+        torch.linalg.vector_norm(B, dim=1)
+    return (num / den).amax(dim=1)
+
+
+def time_steps(fn, steps, warmup):
+    """Warmup, then exactly `steps` calls bracketed by device synchronisation;
+    returns (wall seconds, HIP-event seconds) for the K calls."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, e0.elapsed_time(e1) * 1e-3
+
+
+def cpu_baseline(A, B, nrhs):
+    """The reference's posv (its Auto backend: SIMD Cholesky + scalar solve),
+    compiled from its own sources, on the host cores; best of reps like the
+    reference's bench.py:24-32.  Falls back to the oracle restatement ("port")
+    when the compiled reference is not present."""
+    import numpy as np
+    from oracle import Oracle, available
+    kind = "reference" if available("reference") else "port"
+    orc = Oracle(kind)
+    cores = len(os.sched_getaffinity(0))
+    orc.set_num_threads(cores)
+    Ah = A.cpu().numpy()
+    Bh = B.cpu().numpy()
+    best, total, reps = float("inf"), 0.0, 0
+    while reps < 5 and total < 20.0:
+        t0 = time.perf_counter()
+        orc.posv(Ah, Bh)
+        dt = time.perf_counter() - t0
+        best, total, reps = min(best, dt), total + dt, reps + 1
+    batch = Ah.shape[0]
+    return {"value": batch / best, "unit": "solves/s", "cores": cores, "kind": kind,
+            "sample": f"full workload ({batch} matrices, nrhs={nrhs}), best of {reps} reps"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nrhs", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=131072)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the batched solvers shard by "
+              "splitting the batch, with no exchange between GPUs", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    import tinysolverdx  # noqa: F401  (binds to torch's HIP runtime)
+    from tinysolverdx import device
+
+    n, batch, nrhs = 64, a.batch, a.nrhs
+    log(f"[bench] generating {batch} SPD fp64 n={n} matrices on {torch.cuda.get_device_name(0)}")
+    t0 = time.perf_counter()
+    A, kappa = make_spd(n, batch, 1e1, 1e6, seed=20261205)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    B = torch.randn((batch, n, nrhs), generator=g, device="cuda", dtype=torch.float64)
+    X = torch.empty_like(B)
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f}s")
+
+    # ---- plain FP64 posv (factor + solve in one launch, A and B read, X written)
+    def plain():
+        device.posv_solve(A, B, X, info)
+
+    wall, ev = time_steps(plain, a.steps, a.warmup)
+    bw = backward_error(A, X, B)
+    fails = int((info != 0).sum().item())
+    ok = int((bw <= 1e-14).sum().item())
+    plain_rate = batch * a.steps / wall
+    log(f"[bench] plain fp64 posv: {ev / a.steps * 1e3:.3f} ms/launch, {plain_rate / 1e6:.2f} M solves/s, "
+        f"info>0: {fails}, bwd err <= 1e-14: {ok}/{batch}, max {bw.max().item():.2e}")
+
+    # algorithmic bytes per matrix: read A (8n^2), read B and write X (8n*nrhs each), info (4)
+    bytes_per = 8 * n * n + 2 * 8 * n * nrhs + 4
+    kern_s = ev / a.steps
+    achieved = bytes_per * batch / kern_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_posv<double,64,1> (plain fp64 posv, solve-only)",
+                "bytes_per_matrix": bytes_per}
+
+    value, tier = plain_rate, "plain-fp64"
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"posv n={n} SPD kappa log-uniform [1e1,1e6], batch={batch}, nrhs={nrhs}, fp64 inputs",
+                   "tier": tier, "bwd_err_le_1e-14": f"{ok}/{batch}", "info_gt_0": fails,
+                   "plain_fp64_solves_per_s": round(plain_rate, 1), "mixed_over_plain": 1.0},
+        "roofline": roofline,
+    }
+    if not a.no_cpu:
+        log("[bench] CPU baseline (reference posv on the host cores)")
+        out["cpu_baseline"] = cpu_baseline(A, B, nrhs)
+        log(f"[bench] CPU baseline {out['cpu_baseline']['value']:.3g} solves/s on {out['cpu_baseline']['cores']} cores")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

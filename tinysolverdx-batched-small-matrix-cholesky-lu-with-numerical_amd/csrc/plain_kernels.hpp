@@ -294,7 +294,8 @@ __global__ __launch_bounds__(kWave) void k_getrs(const T* __restrict__ LU, const
   // source row of P*B for position i: undo the swaps from the last one back
   int src = t.i;
   for (int k = n - 1; k >= 0; --k) {
-    const int p = pv[t.g * NB + k];
+    int p = pv[t.g * NB + k];
+    p = (p < 0 || p >= n) ? k : p;  // an out-of-range pivot (invalid input) is read as "no swap"
     src = (src == k) ? p : (src == p ? k : src);
   }
   T r[NB];

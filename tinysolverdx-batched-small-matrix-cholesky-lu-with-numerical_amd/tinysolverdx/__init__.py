@@ -13,8 +13,12 @@ device launch; `tinysolverdx.device` takes torch device tensors without copies.
 """
 import numpy as np
 
-from . import _core
-from ._core import (  # noqa: F401  (re-exported)
+from . import _lib
+
+_lib.bind_runtime()  # before _core pulls in libtsdx_hip.so: one HIP runtime per process
+
+from . import _core  # noqa: E402
+from ._core import (  # noqa: E402,F401  (re-exported)
     has_hip, has_cuda, has_openmp, num_threads, set_num_threads, unit_roundoff,
     chol_backward_error, lu_backward_error, solve_backward_error,
     solve_forward_error, reference_solve,

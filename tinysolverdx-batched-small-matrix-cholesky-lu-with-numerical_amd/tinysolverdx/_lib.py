@@ -4,6 +4,7 @@ Loading fails loudly when the library has not been built: there is no CPU
 fallback anywhere in this package.
 """
 import ctypes as C
+import os
 import pathlib
 
 HERE = pathlib.Path(__file__).resolve().parent
@@ -41,12 +42,28 @@ _RES = {"tsdx_hip_last_error": C.c_char_p}
 _lib = None
 
 
+def bind_runtime():
+    """One HIP runtime per process.  PyTorch-ROCm wheels bundle their own
+    libamdhip64.so.7 and load it under a different file name; if this library
+    loaded /opt/rocm's copy first, torch would bring a second runtime into the
+    process and fail to see the GPU.  Loading torch first (when it is
+    installed) makes libtsdx_hip.so bind to torch's copy, which has the same
+    SONAME.  TSDX_NO_TORCH=1 skips this."""
+    if os.environ.get("TSDX_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def lib():
     """The loaded library (loaded once)."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is not built: run build.py (tinysolverdx-hip has no CPU fallback)")
+        bind_runtime()
         L = C.CDLL(str(LIB_PATH))
         for name, args in SIGNATURES.items():
             f = getattr(L, name)
