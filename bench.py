@@ -36,25 +36,43 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_spd(n, batch, kmin, kmax, seed, chunk=8192):
-    """SPD float64 matrices on the device with prescribed per-matrix kappa."""
+def haar_q(z):
+    """Orthogonal factor of the QR of z with R's diagonal made positive (the
+    reference's matrices._orthogonal), by classical Gram-Schmidt with one
+    re-orthogonalisation pass (CGS2), batched on the device."""
+    import torch
+    m, n, _ = z.shape
+    q = torch.empty_like(z)
+    for j in range(n):
+        v = z[:, :, j]
+        if j:
+            qj = q[:, :, :j]
+            for _ in range(2):
+                c = (qj * v[:, :, None]).sum(dim=1)
+                v = v - torch.matmul(qj, c[:, :, None])[:, :, 0]
+        q[:, :, j] = v / torch.linalg.vector_norm(v, dim=1, keepdim=True)
+    return q
+
+
+def make_spd(n, batch, kmin, kmax, seed, chunk=16384):
+    """SPD float64 matrices on the device, A = Q diag(lambda) Q^T symmetrised,
+    lambda_0 = 1, lambda_{n-1} = 1/kappa, the rest log-uniform in between,
+    kappa log-uniform on [kmin, kmax] per matrix."""
+    import math
     import torch
     g = torch.Generator(device="cuda").manual_seed(seed)
     A = torch.empty((batch, n, n), dtype=torch.float64, device="cuda")
-    kappa = 10.0 ** (torch.rand(batch, generator=g, device="cuda", dtype=torch.float64) *
-                     (torch.log10(torch.tensor(kmax)) - torch.log10(torch.tensor(kmin))).item() +
-                     torch.log10(torch.tensor(kmin)).item())
+    lo, hi = math.log10(kmin), math.log10(kmax)
+    kappa = 10.0 ** (lo + (hi - lo) * torch.rand(batch, generator=g, device="cuda", dtype=torch.float64))
     for s in range(0, batch, chunk):
         e = min(batch, s + chunk)
         m = e - s
-        z = torch.randn((m, n, n), generator=g, device="cuda", dtype=torch.float64)
-        q, r = torch.linalg.qr(z)
-        q = q * torch.sign(torch.diagonal(r, dim1=1, dim2=2)).unsqueeze(1)
+        q = haar_q(torch.randn((m, n, n), generator=g, device="cuda", dtype=torch.float64))
         t = torch.rand((m, n), generator=g, device="cuda", dtype=torch.float64)
         t[:, 0] = 0.0
         t[:, n - 1] = 1.0
-        lam = kappa[s:e, None] ** (-t)  # 1 and 1/kappa at the ends, log-uniform inside
-        a = torch.matmul(q * lam.unsqueeze(1), q.transpose(1, 2).contiguous())
+        lam = kappa[s:e, None] ** (-t)
+        a = torch.matmul(q * lam[:, None, :], q.transpose(1, 2).contiguous())
         A[s:e] = 0.5 * (a + a.transpose(1, 2))
     return A, kappa
 

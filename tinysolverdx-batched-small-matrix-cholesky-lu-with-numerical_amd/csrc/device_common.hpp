@@ -147,6 +147,15 @@ __device__ __forceinline__ T infT() {
   return (T)__builtin_huge_val();
 }
 
+// Returns x unchanged but hides its value from the optimiser.  Used on the
+// lane index at the top of the right-hand-side chunk loops: otherwise LLVM
+// hoists the ~3*NB lane masks of the substitution passes (i == k, i < k, ...)
+// out of the loop and spills them, which costs more than recomputing them.
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // Wave-local LDS hand-off: every kernel here runs one wave per workgroup, so
 // the workgroup barrier only orders the LDS traffic of that wave.
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }

@@ -32,30 +32,49 @@ namespace tsdx_hip {
 // ------------------------------------------------------------ Cholesky
 // r: row i of A on entry, row i of L on exit (entries j > i are garbage).
 // col: per-wave LDS, 2*64 elements.  Returns the group's LAPACK info.
+//
+// Step k: column k (already scaled) goes through LDS, every lane applies
+// r[j] -= l_ik * l_jk.  The update of column k+1 is issued first and the
+// pivot work of column k+1 (broadcast of the diagonal, sqrt, the division of
+// the column) right after it, so that dependent chain overlaps the remaining
+// independent updates of step k instead of following them.
 template <class T, int NB>
 __device__ __forceinline__ int chol_regs(T (&r)[NB], int n, int i, int g, T* col) {
   int info = 0;
-  static_for<0, NB>([&](auto kc) {
+  auto pivot = [&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    if (k < n) {
-      const T akk = bcast<NB, k>(r[k]);
-      const bool bad = !(akk > T(0));
-      info = (bad && info == 0) ? k + 1 : info;
-      const T d = sqrtT(bad ? T(1) : akk);
-      const T l = r[k] / d;
-      r[k] = (i == k) ? d : ((i > k) ? l : r[k]);
-      if constexpr (k + 1 < NB) {
-        T* cb = col + (k & 1) * kWave + g * NB;
-        cb[i] = r[k];
-        wave_sync();
-        static_for<k + 1, NB>([&](auto jc) {
-          constexpr int j = decltype(jc)::value;
-          r[j] = fmaT(-r[k], cb[j], r[j]);
-        });
-      }
+    const T akk = bcast<NB, k>(r[k]);
+    const bool bad = !(akk > T(0));
+    info = (bad && info == 0 && k < n) ? k + 1 : info;
+    const T d = sqrtT(bad ? T(1) : akk);
+    const T l = r[k] / d;
+    r[k] = (i == k) ? d : ((i > k) ? l : r[k]);
+  };
+  pivot(ic<0>{});
+  static_for<0, NB - 1>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k + 1 < n) {
+      T* cb = col + (k & 1) * kWave + g * NB;
+      cb[i] = r[k];
+      wave_sync();
+      r[k + 1] = fmaT(-r[k], cb[k + 1], r[k + 1]);
+      pivot(ic<k + 1>{});
+      static_for<k + 2, NB>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        r[j] = fmaT(-r[k], cb[j], r[j]);
+      });
     }
   });
   return info;
+}
+
+// Reciprocal of the diagonal entry this lane holds (row i of a factor whose
+// diagonal sits in register i): the substitution passes multiply by it.
+template <class T, int NB>
+__device__ __forceinline__ T diag_inv(const T (&r)[NB], int i) {
+  T d = T(1);
+  static_for<0, NB>([&](auto j) { d = (i == j) ? r[j] : d; });
+  return T(1) / d;
 }
 
 // ------------------------------------------------------------------- LU
@@ -102,15 +121,19 @@ __device__ __forceinline__ void lu_regs(T (&r)[NB], int n, int i, int g, T* rowb
 }
 
 // ---------------------------------------------------------------- solves
+// Substitution passes.  dinv is 1 / (this lane's diagonal): y_k = b_k * dinv
+// instead of b_k / d_k keeps the per-step dependence chain short (one extra
+// rounding per step, the same backward stability).
+//
 // Forward pass with L held row-per-lane (lane == row): y_i = (b_i - sum_{k<i}
 // L_ik y_k) / L_ii, updates applied in ascending k as the reference does.
 template <class T, int NB, int NR, bool UNIT>
-__device__ __forceinline__ void forward_rows(const T (&r)[NB], T (&b)[NR], int n, int i) {
+__device__ __forceinline__ void forward_rows(const T (&r)[NB], T (&b)[NR], int n, int i, T dinv) {
   static_for<0, NB>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     if (k < n) {
       static_for<0, NR>([&](auto c) {
-        const T yk = UNIT ? bcast<NB, k>(b[c]) : bcast<NB, k>(b[c] / r[k]);
+        const T yk = UNIT ? bcast<NB, k>(b[c]) : bcast<NB, k>(b[c] * dinv);
         b[c] = (i == k) ? yk : (i > k ? fmaT(-r[k], yk, b[c]) : b[c]);
       });
     }
@@ -120,12 +143,12 @@ __device__ __forceinline__ void forward_rows(const T (&r)[NB], T (&b)[NR], int n
 // Backward pass with an upper factor held row-per-lane (U_ik = r[k] in lane i):
 // column sweep k = n-1..0, x_k = b_k / U_kk, then b_i -= U_ik x_k for i < k.
 template <class T, int NB, int NR>
-__device__ __forceinline__ void backward_rows_upper(const T (&r)[NB], T (&b)[NR], int n, int i) {
+__device__ __forceinline__ void backward_rows_upper(const T (&r)[NB], T (&b)[NR], int n, int i, T dinv) {
   static_for_rev<0, NB>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     if (k < n) {
       static_for<0, NR>([&](auto c) {
-        const T xk = bcast<NB, k>(b[c] / r[k]);
+        const T xk = bcast<NB, k>(b[c] * dinv);
         b[c] = (i == k) ? xk : (i < k ? fmaT(-r[k], xk, b[c]) : b[c]);
       });
     }
@@ -136,7 +159,7 @@ __device__ __forceinline__ void backward_rows_upper(const T (&r)[NB], T (&b)[NR]
 // rows i < k, so blocks of CH rows of L are transposed through LDS.
 // tb: per-wave LDS, 64*CH elements.
 template <class T, int NB, int NR>
-__device__ __forceinline__ void backward_rows_lt(const T (&r)[NB], T (&b)[NR], int n, int i, int g, T* tb) {
+__device__ __forceinline__ void backward_rows_lt(const T (&r)[NB], T (&b)[NR], int n, int i, int g, T* tb, T dinv) {
   constexpr int CH = NB < 8 ? NB : 8;
   T* tg = tb + g * (CH * NB);
   static_for_rev<0, NB / CH>([&](auto qc) {
@@ -152,7 +175,7 @@ __device__ __forceinline__ void backward_rows_lt(const T (&r)[NB], T (&b)[NR], i
         if (k < n) {
           const T lki = tg[(k - c0) * NB + i];
           static_for<0, NR>([&](auto c) {
-            const T xk = bcast<NB, k>(b[c] / r[k]);
+            const T xk = bcast<NB, k>(b[c] * dinv);
             b[c] = (i == k) ? xk : (i < k ? fmaT(-lki, xk, b[c]) : b[c]);
           });
         }
@@ -166,7 +189,7 @@ __device__ __forceinline__ void backward_rows_lt(const T (&r)[NB], T (&b)[NR], i
 // (lane i holds factor row pos, and right-hand-side row i, which is row pos of
 // P*B).  yb: per-wave LDS, 2*64 elements.
 template <class T, int NB, int NR>
-__device__ __forceinline__ void lu_solve_pos(const T (&r)[NB], T (&b)[NR], int n, int pos, int g, T* yb) {
+__device__ __forceinline__ void lu_solve_pos(const T (&r)[NB], T (&b)[NR], int n, int pos, int g, T* yb, T dinv) {
   static_for<0, NB>([&](auto tc) {
     constexpr int t = decltype(tc)::value;
     if (t < n) {
@@ -184,7 +207,7 @@ __device__ __forceinline__ void lu_solve_pos(const T (&r)[NB], T (&b)[NR], int n
     if (t < n) {
       T* yg = yb + (t & 1) * kWave + g * NB;
       if (pos == t) static_for<0, NR>([&](auto c) {
-        b[c] = b[c] / r[t];
+        b[c] = b[c] * dinv;
         yg[c] = b[c];
       });
       wave_sync();
@@ -272,10 +295,12 @@ __global__ __launch_bounds__(kWave) void k_potrs(const T* __restrict__ L, int n,
     T b[NR];
     T* brow = B + (t.m * n + t.i) * (int64_t)nrhs + c0;
     load_rhs<T, NR>(b, brow, nrhs - c0, io);
-    load_row<T, NB>(r, base + (int64_t)t.i * n, n, io, vec != 0);
-    forward_rows<T, NB, NR, false>(r, b, n, t.i);
-    static_for<0, NB>([&](auto k) { r[k] = (io && k < n) ? base[(int64_t)k * n + t.i] : T(0); });
-    backward_rows_upper<T, NB, NR>(r, b, n, t.i);
+    const int li = opaque(t.i);
+    load_row<T, NB>(r, base + (int64_t)li * n, n, io, vec != 0);
+    const T dinv = diag_inv<T, NB>(r, li);
+    forward_rows<T, NB, NR, false>(r, b, n, li, dinv);
+    static_for<0, NB>([&](auto k) { r[k] = (io && k < n) ? base[(int64_t)k * n + li] : T(0); });
+    backward_rows_upper<T, NB, NR>(r, b, n, li, dinv);
     store_rhs<T, NR>(b, brow, nrhs - c0, io);
   }
 }
@@ -300,12 +325,14 @@ __global__ __launch_bounds__(kWave) void k_getrs(const T* __restrict__ LU, const
   }
   T r[NB];
   load_row<T, NB>(r, LU + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
+  const T dinv = diag_inv<T, NB>(r, t.i);
   for (int c0 = 0; c0 < nrhs; c0 += NR) {
     T b[NR];
     load_rhs<T, NR>(b, B + (t.m * n + src) * (int64_t)nrhs + c0, nrhs - c0, io);
     wave_sync();  // every lane has read its source row before any row is rewritten
-    forward_rows<T, NB, NR, true>(r, b, n, t.i);
-    backward_rows_upper<T, NB, NR>(r, b, n, t.i);
+    const int li = opaque(t.i);
+    forward_rows<T, NB, NR, true>(r, b, n, li, T(1));
+    backward_rows_upper<T, NB, NR>(r, b, n, li, dinv);
     store_rhs<T, NR>(b, B + (t.m * n + t.i) * (int64_t)nrhs + c0, nrhs - c0, io);
   }
 }
@@ -327,12 +354,14 @@ __global__ __launch_bounds__(kWave) void k_posv(const T* __restrict__ A, T* __re
   const int inf = chol_regs<T, NB>(r, n, t.i, t.g, col);
   if (Lout && io) store_row<T, NB>(r, Lout + t.m * nn + (int64_t)t.i * n, n, vec != 0, true, t.i);
   if (t.live && t.i == 0 && info) info[t.m] = inf;
+  const T dinv = diag_inv<T, NB>(r, t.i);
   for (int c0 = 0; c0 < nrhs; c0 += NR) {
     T b[NR];
     const int64_t off = (t.m * n + t.i) * (int64_t)nrhs + c0;
     load_rhs<T, NR>(b, Bin + off, nrhs - c0, io);
-    forward_rows<T, NB, NR, false>(r, b, n, t.i);
-    backward_rows_lt<T, NB, NR>(r, b, n, t.i, t.g, tb);
+    const int li = opaque(t.i);
+    forward_rows<T, NB, NR, false>(r, b, n, li, dinv);
+    backward_rows_lt<T, NB, NR>(r, b, n, li, t.g, tb, dinv);
     store_rhs<T, NR>(b, X + off, nrhs - c0, io);
   }
 }
@@ -355,11 +384,15 @@ __global__ __launch_bounds__(kWave) void k_gesv(const T* __restrict__ A, T* __re
     if (piv) piv[t.m * n + t.i] = mypiv;
   }
   if (t.live && t.i == 0 && info) info[t.m] = inf;
+  // this lane's row of U sits at position pos: its pivot is r[pos]
+  T upiv = T(1);
+  static_for<0, NB>([&](auto j) { upiv = (pos == j) ? r[j] : upiv; });
+  const T dinv = T(1) / upiv;
   for (int c0 = 0; c0 < nrhs; c0 += NR) {
     T b[NR];
     load_rhs<T, NR>(b, Bin + (t.m * n + t.i) * (int64_t)nrhs + c0, nrhs - c0, io);
     wave_sync();
-    lu_solve_pos<T, NB, NR>(r, b, n, pos, t.g, rowb);
+    lu_solve_pos<T, NB, NR>(r, b, n, opaque(pos), t.g, rowb, dinv);
     store_rhs<T, NR>(b, X + (t.m * n + pos) * (int64_t)nrhs + c0, nrhs - c0, io);
   }
 }
