@@ -156,9 +156,22 @@ __device__ __forceinline__ int opaque(int x) {
   return x;
 }
 
-// Wave-local LDS hand-off: every kernel here runs one wave per workgroup, so
-// the workgroup barrier only orders the LDS traffic of that wave.
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+// Waves per workgroup.  Every wave works on its own matrices with its own LDS
+// region; several waves per workgroup only raise the number of resident waves
+// per CU (workgroup slots are a separate, smaller limit than wave slots).
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWave * kWavesPerBlock;
+
+// Wave-local LDS hand-off: orders this wave's LDS writes before its later LDS
+// reads (other lanes of the same wave read what a lane wrote).  The fences stop
+// the compiler from moving LDS accesses across; the wait makes the writes land.
+// No s_barrier: the other waves of the workgroup are never involved.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0), vmcnt/expcnt untouched
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // ------------------------------------------------------- row load / store
 template <class T>

@@ -231,48 +231,53 @@ __device__ __forceinline__ void store_rhs(const T (&b)[NR], T* __restrict__ p, i
 }
 
 // ---------------------------------------------------------------- kernels
+// Thread -> (wave w of the block, group g of the wave, row i of the group);
+// matrix m = (block * waves + w) * groups + g.
 struct Tile {
-  int lane, g, i;
+  int lane, w, g, i;
   int64_t m;
   bool live;
 };
 template <int NB>
 __device__ __forceinline__ Tile tile_of(int64_t batch) {
   Tile t;
-  t.lane = threadIdx.x;
+  t.lane = threadIdx.x % kWave;
+  t.w = threadIdx.x / kWave;
   t.g = t.lane / NB;
   t.i = t.lane % NB;
-  t.m = (int64_t)blockIdx.x * (kWave / NB) + t.g;
+  t.m = ((int64_t)blockIdx.x * kWavesPerBlock + t.w) * (kWave / NB) + t.g;
   t.live = t.m < batch;
   return t;
 }
 
 template <class T, int NB>
-__global__ __launch_bounds__(kWave) void k_potrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ info, int vec) {
-  __shared__ T col[2 * kWave];
+__global__ __launch_bounds__(kBlock) void k_potrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ info, int vec) {
+  __shared__ T col_s[kWavesPerBlock][2 * kWave];
   const Tile t = tile_of<NB>(batch);
+  [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
   T* row = A + t.m * nn + (int64_t)t.i * n;
   const bool io = t.live && t.i < n;
   T r[NB];
   load_row<T, NB>(r, row, n, io, vec != 0);
-  const int inf = chol_regs<T, NB>(r, n, t.i, t.g, col);
+  const int inf = chol_regs<T, NB>(r, n, t.i, t.g, col_s[wv]);
   if (io) store_row<T, NB>(r, row, n, vec != 0, true, t.i);
   if (t.live && t.i == 0 && info) info[t.m] = inf;
 }
 
 template <class T, int NB, bool ZTILE>
-__global__ __launch_bounds__(kWave) void k_getrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ piv,
+__global__ __launch_bounds__(kBlock) void k_getrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ piv,
                                                  int* __restrict__ info, int vec) {
-  __shared__ T rowb[2 * kWave];
+  __shared__ T rowb_s[kWavesPerBlock][2 * kWave];
   const Tile t = tile_of<NB>(batch);
+  [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
   T* base = A + t.m * nn;
   const bool io = t.live && t.i < n;
   T r[NB];
   load_row<T, NB>(r, base + (int64_t)t.i * n, n, io, vec != 0);
   int pos, mypiv, inf;
-  lu_regs<T, NB, ZTILE>(r, n, t.i, t.g, rowb, pos, mypiv, inf);
+  lu_regs<T, NB, ZTILE>(r, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
   if (io) {
     store_row<T, NB>(r, base + (int64_t)pos * n, n, vec != 0, false, 0);
     piv[t.m * n + t.i] = mypiv;
@@ -284,9 +289,10 @@ __global__ __launch_bounds__(kWave) void k_getrf(T* __restrict__ A, int n, int64
 // L (lane i loads column i, which is row i of L^T; loads are coalesced across
 // lanes because lane i reads element i of each row).
 template <class T, int NB, int NR>
-__global__ __launch_bounds__(kWave) void k_potrs(const T* __restrict__ L, int n, int64_t batch, T* __restrict__ B,
+__global__ __launch_bounds__(kBlock) void k_potrs(const T* __restrict__ L, int n, int64_t batch, T* __restrict__ B,
                                                  int nrhs, int vec) {
   const Tile t = tile_of<NB>(batch);
+  [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
   const T* base = L + t.m * nn;
   const bool io = t.live && t.i < n;
@@ -308,12 +314,14 @@ __global__ __launch_bounds__(kWave) void k_potrs(const T* __restrict__ L, int n,
 // getrs: LU from memory (row t at lane t); right-hand-side row t is row
 // perm[t] of B where perm applies the sequential swaps piv[0..n-1].
 template <class T, int NB, int NR>
-__global__ __launch_bounds__(kWave) void k_getrs(const T* __restrict__ LU, const int* __restrict__ piv, int n,
+__global__ __launch_bounds__(kBlock) void k_getrs(const T* __restrict__ LU, const int* __restrict__ piv, int n,
                                                  int64_t batch, T* __restrict__ B, int nrhs, int vec) {
-  __shared__ int pv[kWave];
+  __shared__ int pv_s[kWavesPerBlock][kWave];
   const Tile t = tile_of<NB>(batch);
+  [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
   const bool io = t.live && t.i < n;
+  int* pv = pv_s[wv];
   pv[t.lane] = io ? piv[t.m * n + t.i] : 0;
   wave_sync();
   // source row of P*B for position i: undo the swaps from the last one back
@@ -340,18 +348,19 @@ __global__ __launch_bounds__(kWave) void k_getrs(const T* __restrict__ LU, const
 // posv: potrf + potrs with the factor kept in registers.  Lout == nullptr
 // leaves A untouched (solve-only use); otherwise the factor is written there.
 template <class T, int NB, int NR>
-__global__ __launch_bounds__(kWave) void k_posv(const T* __restrict__ A, T* __restrict__ Lout, int n, int64_t batch,
+__global__ __launch_bounds__(kBlock) void k_posv(const T* __restrict__ A, T* __restrict__ Lout, int n, int64_t batch,
                                                 const T* __restrict__ Bin, T* __restrict__ X, int nrhs,
                                                 int* __restrict__ info, int vec) {
   constexpr int CH = NB < 8 ? NB : 8;
-  __shared__ T col[2 * kWave];
-  __shared__ T tb[kWave * CH];
+  __shared__ T col_s[kWavesPerBlock][2 * kWave];
+  __shared__ T tb_s[kWavesPerBlock][kWave * CH];
   const Tile t = tile_of<NB>(batch);
+  [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
   const bool io = t.live && t.i < n;
   T r[NB];
   load_row<T, NB>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
-  const int inf = chol_regs<T, NB>(r, n, t.i, t.g, col);
+  const int inf = chol_regs<T, NB>(r, n, t.i, t.g, col_s[wv]);
   if (Lout && io) store_row<T, NB>(r, Lout + t.m * nn + (int64_t)t.i * n, n, vec != 0, true, t.i);
   if (t.live && t.i == 0 && info) info[t.m] = inf;
   const T dinv = diag_inv<T, NB>(r, t.i);
@@ -361,24 +370,25 @@ __global__ __launch_bounds__(kWave) void k_posv(const T* __restrict__ A, T* __re
     load_rhs<T, NR>(b, Bin + off, nrhs - c0, io);
     const int li = opaque(t.i);
     forward_rows<T, NB, NR, false>(r, b, n, li, dinv);
-    backward_rows_lt<T, NB, NR>(r, b, n, li, t.g, tb, dinv);
+    backward_rows_lt<T, NB, NR>(r, b, n, li, t.g, tb_s[wv], dinv);
     store_rhs<T, NR>(b, X + off, nrhs - c0, io);
   }
 }
 
 // gesv: getrf + getrs with the factor kept in registers.
 template <class T, int NB, int NR, bool ZTILE>
-__global__ __launch_bounds__(kWave) void k_gesv(const T* __restrict__ A, T* __restrict__ LUout, int n, int64_t batch,
+__global__ __launch_bounds__(kBlock) void k_gesv(const T* __restrict__ A, T* __restrict__ LUout, int n, int64_t batch,
                                                 const T* __restrict__ Bin, T* __restrict__ X, int nrhs,
                                                 int* __restrict__ piv, int* __restrict__ info, int vec) {
-  __shared__ T rowb[2 * kWave];
+  __shared__ T rowb_s[kWavesPerBlock][2 * kWave];
   const Tile t = tile_of<NB>(batch);
+  [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
   const bool io = t.live && t.i < n;
   T r[NB];
   load_row<T, NB>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
   int pos, mypiv, inf;
-  lu_regs<T, NB, ZTILE>(r, n, t.i, t.g, rowb, pos, mypiv, inf);
+  lu_regs<T, NB, ZTILE>(r, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
   if (io) {
     if (LUout) store_row<T, NB>(r, LUout + t.m * nn + (int64_t)pos * n, n, vec != 0, false, 0);
     if (piv) piv[t.m * n + t.i] = mypiv;
@@ -392,7 +402,7 @@ __global__ __launch_bounds__(kWave) void k_gesv(const T* __restrict__ A, T* __re
     T b[NR];
     load_rhs<T, NR>(b, Bin + (t.m * n + t.i) * (int64_t)nrhs + c0, nrhs - c0, io);
     wave_sync();
-    lu_solve_pos<T, NB, NR>(r, b, n, opaque(pos), t.g, rowb, dinv);
+    lu_solve_pos<T, NB, NR>(r, b, n, opaque(pos), t.g, rowb_s[wv], dinv);
     store_rhs<T, NR>(b, X + (t.m * n + pos) * (int64_t)nrhs + c0, nrhs - c0, io);
   }
 }
