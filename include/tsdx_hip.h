@@ -29,6 +29,7 @@
 #ifndef TSDX_HIP_H
 #define TSDX_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -93,6 +94,33 @@ int tsdx_hip_gesv_solve_f32(const float *A, int n, int64_t batch, const float *B
                             int *info, void *stream);
 int tsdx_hip_gesv_solve_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs, int *piv,
                             int *info, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Mixed-precision posv with iterative refinement (float64 A, B, X).  No
+ * reference counterpart: the reference has only the plain posv_batched
+ * (batched.hpp:271-275).  Each matrix is factored in the low precision
+ * `factor`, each right-hand side refined with float64 residuals until the
+ * normwise backward error ||b - Ax|| / (||A||_F ||x|| + ||b||) <= tol, at most
+ * max_iter corrections; matrices whose low-precision factorization fails or
+ * whose refinement does not reach tol are solved again by the plain float64
+ * posv (the fallback), so every X meets tol unless A itself is not SPD in
+ * float64 (then info > 0, as for tsdx_hip_posv_f64).
+ *   factor: TSDX_HIP_FACTOR_FP32 / _BF16 / _FP16 (low precision + refinement),
+ *           _FP64 (plain float64 solve for every matrix), _AUTO (FP32).
+ *   stat:   (batch) or NULL: corrections used, or -1 for a matrix that took
+ *           the float64 fallback.
+ *   work:   device workspace of tsdx_hip_posv_ir_workspace_size(batch) bytes
+ *           (the fallback list); calls sharing one workspace must not overlap.
+ * ------------------------------------------------------------------------ */
+#define TSDX_HIP_FACTOR_AUTO 0
+#define TSDX_HIP_FACTOR_FP32 1
+#define TSDX_HIP_FACTOR_BF16 2
+#define TSDX_HIP_FACTOR_FP16 3
+#define TSDX_HIP_FACTOR_FP64 4
+
+size_t tsdx_hip_posv_ir_workspace_size(int64_t batch);
+int tsdx_hip_posv_ir_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs, int factor,
+                         int max_iter, double tol, int *info, int *stat, void *work, size_t work_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Host-pointer entry points (synchronous).  Shapes of the reference's CUDA

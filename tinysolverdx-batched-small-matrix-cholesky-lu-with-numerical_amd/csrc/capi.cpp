@@ -14,6 +14,7 @@
 #include "../../include/tsdx_hip.h"
 #include "capi_internal.hpp"
 #include "launch.hpp"
+#include "mixed_api.hpp"
 
 namespace tsdx_hip {
 
@@ -141,6 +142,40 @@ void gesv(const T* A, T* LUout, int n, int64_t batch, const T* B, T* X, int nrhs
                                                                      n <= kLuTileMaxN, s);
                         }),
             "gesv launch");
+}
+
+// ------------------------------------------------------ refinement tier
+size_t ir_workspace_size(int64_t batch) {
+  const size_t b = batch < 0 ? 0 : (size_t)batch;
+  return 256 + ((b * sizeof(int) + 255) / 256) * 256;
+}
+
+void posv_ir(const double* A, int n, int64_t batch, const double* B, double* X, int nrhs, int factor, int max_iter,
+             double tol, int* info, int* stat, void* work, size_t work_bytes, hipStream_t s) {
+  check_dims(n, batch);
+  if (nrhs < 0) throw InvalidArg("negative nrhs");
+  if (factor < TSDX_HIP_FACTOR_AUTO || factor > TSDX_HIP_FACTOR_FP64) throw InvalidArg("unknown factor precision");
+  if (max_iter < 0 || max_iter > 1000) throw InvalidArg("max_iter must be in [0, 1000]");
+  if (!(tol > 0.0)) throw InvalidArg("tol must be positive");
+  if (batch == 0) return;
+  if (!A || (nrhs > 0 && (!B || !X))) throw InvalidArg("A, B and X must not be null");
+  if (!work || work_bytes < ir_workspace_size(batch)) throw InvalidArg("workspace missing or too small");
+  require_device();
+  IrArgs a;
+  a.A = A;
+  a.B = B;
+  a.X = X;
+  a.n = n;
+  a.batch = batch;
+  a.nrhs = nrhs;
+  a.max_iter = max_iter;
+  a.tol = tol;
+  a.info = info;
+  a.stat = stat;
+  a.fcount = static_cast<int*>(work);
+  a.flist = reinterpret_cast<int*>(static_cast<char*>(work) + 256);
+  const int low = factor == TSDX_HIP_FACTOR_AUTO ? TSDX_HIP_FACTOR_FP32 : factor;
+  hip_check(launch_posv_ir(a, low, s), "posv_ir launch");
 }
 
 // -------------------------------------------------------- host staging
@@ -374,6 +409,13 @@ int tsdx_hip_gesv_solve_f32(const float* A, int n, int64_t batch, const float* B
 int tsdx_hip_gesv_solve_f64(const double* A, int n, int64_t batch, const double* B, double* X, int nrhs, int* piv,
                             int* info, void* s) {
   return guarded([&] { gesv<double>(A, nullptr, n, batch, B, X, nrhs, piv, info, false, S(s)); });
+}
+
+size_t tsdx_hip_posv_ir_workspace_size(int64_t batch) { return ir_workspace_size(batch); }
+int tsdx_hip_posv_ir_f64(const double* A, int n, int64_t batch, const double* B, double* X, int nrhs, int factor,
+                         int max_iter, double tol, int* info, int* stat, void* work, size_t work_bytes, void* s) {
+  return guarded(
+      [&] { posv_ir(A, n, batch, B, X, nrhs, factor, max_iter, tol, info, stat, work, work_bytes, S(s)); });
 }
 
 int tsdx_hip_cholesky_batched_f32(float* A, int n, int64_t batch, int* info) {

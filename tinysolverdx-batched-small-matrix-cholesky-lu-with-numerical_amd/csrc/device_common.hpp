@@ -156,11 +156,14 @@ __device__ __forceinline__ int opaque(int x) {
   return x;
 }
 
-// Waves per workgroup.  Every wave works on its own matrices with its own LDS
-// region; several waves per workgroup only raise the number of resident waves
-// per CU (workgroup slots are a separate, smaller limit than wave slots).
-constexpr int kWavesPerBlock = 4;
-constexpr int kBlock = kWave * kWavesPerBlock;
+// Waves per workgroup for a size bucket.  Every wave works on its own
+// matrices with its own LDS region, so this only changes how waves are packed
+// into workgroups.  Measured on MI355X: four-wave workgroups are faster for
+// the small buckets (many short waves), one-wave workgroups for n > 16.
+template <int NB>
+constexpr int waves_per_block() {
+  return NB <= 16 ? 4 : 1;
+}
 
 // Wave-local LDS hand-off: orders this wave's LDS writes before its later LDS
 // reads (other lanes of the same wave read what a lane wrote).  The fences stop

@@ -9,27 +9,31 @@ namespace tsdx_hip {
 
 template <int NB>
 inline dim3 grid_for(int64_t batch) {
-  constexpr int64_t per_block = (int64_t)kWavesPerBlock * (kWave / NB);
+  constexpr int64_t per_block = (int64_t)waves_per_block<NB>() * (kWave / NB);
   return dim3((unsigned)((batch + per_block - 1) / per_block));
+}
+template <int NB>
+inline dim3 block_for() {
+  return dim3(kWave * waves_per_block<NB>());
 }
 
 template <class T, int NB>
 hipError_t launch_potrf(T* A, int n, int64_t batch, int* info, int vec, hipStream_t s) {
-  hipLaunchKernelGGL((k_potrf<T, NB>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, n, batch, info, vec);
+  hipLaunchKernelGGL((k_potrf<T, NB>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, info, vec);
   return hipGetLastError();
 }
 
 template <class T, int NB>
 hipError_t launch_getrf(T* A, int n, int64_t batch, int* piv, int* info, int vec, bool ztile, hipStream_t s) {
   if constexpr (NB <= 16) {  // n <= 16: always the zero-pivot tile rule
-    hipLaunchKernelGGL((k_getrf<T, NB, true>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, n, batch, piv, info, vec);
+    hipLaunchKernelGGL((k_getrf<T, NB, true>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
   } else if constexpr (NB == 64) {  // n >= 33: always the skip rule
-    hipLaunchKernelGGL((k_getrf<T, NB, false>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, n, batch, piv, info, vec);
+    hipLaunchKernelGGL((k_getrf<T, NB, false>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
   } else {
     if (ztile)
-      hipLaunchKernelGGL((k_getrf<T, NB, true>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, n, batch, piv, info, vec);
+      hipLaunchKernelGGL((k_getrf<T, NB, true>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
     else
-      hipLaunchKernelGGL((k_getrf<T, NB, false>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, n, batch, piv, info, vec);
+      hipLaunchKernelGGL((k_getrf<T, NB, false>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
   }
   return hipGetLastError();
 }
@@ -37,18 +41,18 @@ hipError_t launch_getrf(T* A, int n, int64_t batch, int* piv, int* info, int vec
 template <class T, int NB>
 hipError_t launch_potrs(const T* L, int n, int64_t batch, T* B, int nrhs, int vec, hipStream_t s) {
   if (nrhs == 1)
-    hipLaunchKernelGGL((k_potrs<T, NB, 1>), grid_for<NB>(batch), dim3(kBlock), 0, s, L, n, batch, B, nrhs, vec);
+    hipLaunchKernelGGL((k_potrs<T, NB, 1>), grid_for<NB>(batch), block_for<NB>(), 0, s, L, n, batch, B, nrhs, vec);
   else
-    hipLaunchKernelGGL((k_potrs<T, NB, 4>), grid_for<NB>(batch), dim3(kBlock), 0, s, L, n, batch, B, nrhs, vec);
+    hipLaunchKernelGGL((k_potrs<T, NB, 4>), grid_for<NB>(batch), block_for<NB>(), 0, s, L, n, batch, B, nrhs, vec);
   return hipGetLastError();
 }
 
 template <class T, int NB>
 hipError_t launch_getrs(const T* LU, const int* piv, int n, int64_t batch, T* B, int nrhs, int vec, hipStream_t s) {
   if (nrhs == 1)
-    hipLaunchKernelGGL((k_getrs<T, NB, 1>), grid_for<NB>(batch), dim3(kBlock), 0, s, LU, piv, n, batch, B, nrhs, vec);
+    hipLaunchKernelGGL((k_getrs<T, NB, 1>), grid_for<NB>(batch), block_for<NB>(), 0, s, LU, piv, n, batch, B, nrhs, vec);
   else
-    hipLaunchKernelGGL((k_getrs<T, NB, 4>), grid_for<NB>(batch), dim3(kBlock), 0, s, LU, piv, n, batch, B, nrhs, vec);
+    hipLaunchKernelGGL((k_getrs<T, NB, 4>), grid_for<NB>(batch), block_for<NB>(), 0, s, LU, piv, n, batch, B, nrhs, vec);
   return hipGetLastError();
 }
 
@@ -56,10 +60,10 @@ template <class T, int NB>
 hipError_t launch_posv(const T* A, T* Lout, int n, int64_t batch, const T* B, T* X, int nrhs, int* info, int vec,
                        hipStream_t s) {
   if (nrhs <= 1)
-    hipLaunchKernelGGL((k_posv<T, NB, 1>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, Lout, n, batch, B, X, nrhs, info,
+    hipLaunchKernelGGL((k_posv<T, NB, 1>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, Lout, n, batch, B, X, nrhs, info,
                        vec);
   else
-    hipLaunchKernelGGL((k_posv<T, NB, 4>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, Lout, n, batch, B, X, nrhs, info,
+    hipLaunchKernelGGL((k_posv<T, NB, 4>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, Lout, n, batch, B, X, nrhs, info,
                        vec);
   return hipGetLastError();
 }
@@ -68,17 +72,17 @@ template <class T, int NB, int NR>
 static void gesv_go(const T* A, T* LUout, int n, int64_t batch, const T* B, T* X, int nrhs, int* piv, int* info,
                     int vec, bool ztile, hipStream_t s) {
   if constexpr (NB <= 16) {
-    hipLaunchKernelGGL((k_gesv<T, NB, NR, true>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, LUout, n, batch, B, X,
+    hipLaunchKernelGGL((k_gesv<T, NB, NR, true>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, LUout, n, batch, B, X,
                        nrhs, piv, info, vec);
   } else if constexpr (NB == 64) {
-    hipLaunchKernelGGL((k_gesv<T, NB, NR, false>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, LUout, n, batch, B, X,
+    hipLaunchKernelGGL((k_gesv<T, NB, NR, false>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, LUout, n, batch, B, X,
                        nrhs, piv, info, vec);
   } else {
     if (ztile)
-      hipLaunchKernelGGL((k_gesv<T, NB, NR, true>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, LUout, n, batch, B, X,
+      hipLaunchKernelGGL((k_gesv<T, NB, NR, true>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, LUout, n, batch, B, X,
                          nrhs, piv, info, vec);
     else
-      hipLaunchKernelGGL((k_gesv<T, NB, NR, false>), grid_for<NB>(batch), dim3(kBlock), 0, s, A, LUout, n, batch, B, X,
+      hipLaunchKernelGGL((k_gesv<T, NB, NR, false>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, LUout, n, batch, B, X,
                          nrhs, piv, info, vec);
   }
 }

@@ -1,0 +1,27 @@
+// Host-side declaration of the refinement-tier launcher (mixed_posv.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tsdx_hip {
+
+struct IrArgs {
+  const double* A;
+  const double* B;
+  double* X;
+  int n;
+  int64_t batch;
+  int nrhs;
+  int max_iter;
+  double tol;
+  int* info;    // (batch) or null
+  int* stat;    // (batch) or null: corrections used, or -1 when sent to the float64 solve
+  int* flist;   // (batch) matrices handed to the float64 solve
+  int* fcount;  // (1) their number
+};
+
+// low: 1 float, 2 bfloat16, 3 half (refinement), 4 float64 direct for all.
+hipError_t launch_posv_ir(const IrArgs& a, int low, hipStream_t s);
+
+}  // namespace tsdx_hip

@@ -1,0 +1,33 @@
+// Launchers of the mixed-precision refinement tier (mixed_kernels.hpp).
+#include "capi_internal.hpp"
+#include "launch_impl.hpp"
+#include "mixed_kernels.hpp"
+
+namespace tsdx_hip {
+
+template <int NB>
+static hipError_t launch_ir(const IrArgs& a, int low, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, a.fcount);
+  if (low != 4) {
+    hipLaunchKernelGGL((k_posv_ir<NB>), grid_for<NB>(a.batch), block_for<NB>(), 0, s, a, low);
+  } else {  // float64 direct for every matrix: the list is 0..batch-1
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((a.batch + 255) / 256)), dim3(256), 0, s, a.flist, a.fcount, a.batch);
+  }
+  const dim3 g = grid_for<NB>(a.batch);
+  const dim3 gg(g.x < 16384u ? g.x : 16384u);
+  if (a.nrhs <= 1)
+    hipLaunchKernelGGL((k_posv_gather<NB, 1>), gg, block_for<NB>(), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_posv_gather<NB, 4>), gg, block_for<NB>(), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_posv_ir(const IrArgs& a, int low, hipStream_t s) {
+  if (a.n <= 4) return launch_ir<4>(a, low, s);
+  if (a.n <= 8) return launch_ir<8>(a, low, s);
+  if (a.n <= 16) return launch_ir<16>(a, low, s);
+  if (a.n <= 32) return launch_ir<32>(a, low, s);
+  return launch_ir<64>(a, low, s);
+}
+
+}  // namespace tsdx_hip

@@ -245,14 +245,14 @@ __device__ __forceinline__ Tile tile_of(int64_t batch) {
   t.w = threadIdx.x / kWave;
   t.g = t.lane / NB;
   t.i = t.lane % NB;
-  t.m = ((int64_t)blockIdx.x * kWavesPerBlock + t.w) * (kWave / NB) + t.g;
+  t.m = ((int64_t)blockIdx.x * waves_per_block<NB>() + t.w) * (kWave / NB) + t.g;
   t.live = t.m < batch;
   return t;
 }
 
 template <class T, int NB>
-__global__ __launch_bounds__(kBlock) void k_potrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ info, int vec) {
-  __shared__ T col_s[kWavesPerBlock][2 * kWave];
+__global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_potrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ info, int vec) {
+  __shared__ T col_s[waves_per_block<NB>()][2 * kWave];
   const Tile t = tile_of<NB>(batch);
   [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
@@ -266,9 +266,9 @@ __global__ __launch_bounds__(kBlock) void k_potrf(T* __restrict__ A, int n, int6
 }
 
 template <class T, int NB, bool ZTILE>
-__global__ __launch_bounds__(kBlock) void k_getrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ piv,
+__global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_getrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ piv,
                                                  int* __restrict__ info, int vec) {
-  __shared__ T rowb_s[kWavesPerBlock][2 * kWave];
+  __shared__ T rowb_s[waves_per_block<NB>()][2 * kWave];
   const Tile t = tile_of<NB>(batch);
   [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_getrf(T* __restrict__ A, int n, int6
 // L (lane i loads column i, which is row i of L^T; loads are coalesced across
 // lanes because lane i reads element i of each row).
 template <class T, int NB, int NR>
-__global__ __launch_bounds__(kBlock) void k_potrs(const T* __restrict__ L, int n, int64_t batch, T* __restrict__ B,
+__global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_potrs(const T* __restrict__ L, int n, int64_t batch, T* __restrict__ B,
                                                  int nrhs, int vec) {
   const Tile t = tile_of<NB>(batch);
   [[maybe_unused]] const int wv = t.w;
@@ -314,9 +314,9 @@ __global__ __launch_bounds__(kBlock) void k_potrs(const T* __restrict__ L, int n
 // getrs: LU from memory (row t at lane t); right-hand-side row t is row
 // perm[t] of B where perm applies the sequential swaps piv[0..n-1].
 template <class T, int NB, int NR>
-__global__ __launch_bounds__(kBlock) void k_getrs(const T* __restrict__ LU, const int* __restrict__ piv, int n,
+__global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_getrs(const T* __restrict__ LU, const int* __restrict__ piv, int n,
                                                  int64_t batch, T* __restrict__ B, int nrhs, int vec) {
-  __shared__ int pv_s[kWavesPerBlock][kWave];
+  __shared__ int pv_s[waves_per_block<NB>()][kWave];
   const Tile t = tile_of<NB>(batch);
   [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
@@ -348,12 +348,12 @@ __global__ __launch_bounds__(kBlock) void k_getrs(const T* __restrict__ LU, cons
 // posv: potrf + potrs with the factor kept in registers.  Lout == nullptr
 // leaves A untouched (solve-only use); otherwise the factor is written there.
 template <class T, int NB, int NR>
-__global__ __launch_bounds__(kBlock) void k_posv(const T* __restrict__ A, T* __restrict__ Lout, int n, int64_t batch,
+__global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_posv(const T* __restrict__ A, T* __restrict__ Lout, int n, int64_t batch,
                                                 const T* __restrict__ Bin, T* __restrict__ X, int nrhs,
                                                 int* __restrict__ info, int vec) {
   constexpr int CH = NB < 8 ? NB : 8;
-  __shared__ T col_s[kWavesPerBlock][2 * kWave];
-  __shared__ T tb_s[kWavesPerBlock][kWave * CH];
+  __shared__ T col_s[waves_per_block<NB>()][2 * kWave];
+  __shared__ T tb_s[waves_per_block<NB>()][kWave * CH];
   const Tile t = tile_of<NB>(batch);
   [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;
@@ -377,10 +377,10 @@ __global__ __launch_bounds__(kBlock) void k_posv(const T* __restrict__ A, T* __r
 
 // gesv: getrf + getrs with the factor kept in registers.
 template <class T, int NB, int NR, bool ZTILE>
-__global__ __launch_bounds__(kBlock) void k_gesv(const T* __restrict__ A, T* __restrict__ LUout, int n, int64_t batch,
+__global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_gesv(const T* __restrict__ A, T* __restrict__ LUout, int n, int64_t batch,
                                                 const T* __restrict__ Bin, T* __restrict__ X, int nrhs,
                                                 int* __restrict__ piv, int* __restrict__ info, int vec) {
-  __shared__ T rowb_s[kWavesPerBlock][2 * kWave];
+  __shared__ T rowb_s[waves_per_block<NB>()][2 * kWave];
   const Tile t = tile_of<NB>(batch);
   [[maybe_unused]] const int wv = t.w;
   const int64_t nn = (int64_t)n * n;

@@ -37,7 +37,11 @@ for _t in ("f32", "f64"):
         f"tsdx_hip_posv_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp],
         f"tsdx_hip_gesv_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp, _vp],
     })
-_RES = {"tsdx_hip_last_error": C.c_char_p}
+SIGNATURES["tsdx_hip_posv_ir_workspace_size"] = [_i64]
+SIGNATURES["tsdx_hip_posv_ir_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, C.c_size_t, _vp]
+_RES = {"tsdx_hip_last_error": C.c_char_p, "tsdx_hip_posv_ir_workspace_size": C.c_size_t}
+
+FACTOR = {"auto": 0, "fp32": 1, "bf16": 2, "fp16": 3, "fp64": 4}
 
 _lib = None
 

@@ -111,3 +111,30 @@ def gesv_solve(A, B, X, piv=None, info=None):
     _lib.call(f"tsdx_hip_gesv_solve_{_dt(A)}", _chk(A, "A"), n, batch, _chk(B, "B", A.dtype), _chk(X, "X", A.dtype),
               B.shape[-1], _chk(piv, "piv", torch.int32), _chk(info, "info", torch.int32), _stream(A))
     return X, piv, info
+
+
+def posv_ir_workspace(batch, device="cuda"):
+    """A device workspace for posv_ir (uint8 tensor)."""
+    import torch
+    nbytes = _lib.lib().tsdx_hip_posv_ir_workspace_size(batch)
+    return torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def posv_ir(A, B, X, factor="fp32", max_iter=10, tol=1e-14, info=None, stat=None, work=None):
+    """Mixed-precision posv with float64 iterative refinement (float64 A, B, X).
+
+    factor: "fp32", "bf16", "fp16" (low-precision factor + refinement), "fp64"
+    (plain float64 solve for every matrix) or "auto".  Matrices that do not
+    reach tol by refinement are solved again in float64.  stat (int32) gets
+    the corrections used per matrix, -1 for the float64 fallback."""
+    import torch
+    batch, n = _shape(A)
+    if A.dtype != torch.float64:
+        raise TypeError("posv_ir takes float64 A, B and X")
+    if work is None:
+        work = posv_ir_workspace(batch, A.device)
+    _lib.call("tsdx_hip_posv_ir_f64", _chk(A, "A", torch.float64), n, batch, _chk(B, "B", torch.float64),
+              _chk(X, "X", torch.float64), B.shape[-1], _lib.FACTOR[factor], int(max_iter), float(tol),
+              _chk(info, "info", torch.int32), _chk(stat, "stat", torch.int32), _chk(work, "work"), work.numel(),
+              _stream(A))
+    return X, info, stat
