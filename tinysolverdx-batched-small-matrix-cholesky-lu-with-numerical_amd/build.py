@@ -48,6 +48,18 @@ def _run(cmd, log):
     return out
 
 
+def _includes(path, seen=None):
+    """The unit and every local header it includes, recursively."""
+    import re
+    seen = set() if seen is None else seen
+    if path in seen or not path.exists():
+        return seen
+    seen.add(path)
+    for inc in re.findall(r'#include\s+"([^"]+)"', path.read_text()):
+        _includes((path.parent / inc).resolve(), seen)
+    return seen
+
+
 def _stale(obj, deps):
     if not obj.exists():
         return True
@@ -64,14 +76,13 @@ def build(jobs=None, incremental=False, verbose=False):
     """Compiles every HIP unit for gfx950, links libtsdx_hip.so and the _core module."""
     jobs = jobs or max(1, min(os.cpu_count() or 1, 16))
     BUILD.mkdir(exist_ok=True)
-    headers = sorted(CSRC.glob("*.hpp")) + [ROOT / "include" / "tsdx_hip.h"]
     units = hip_units()
     objs = []
     todo = []
     for u in units:
         o = BUILD / (u.name + ".o")
         objs.append(o)
-        if not incremental or _stale(o, [u] + headers):
+        if not incremental or _stale(o, sorted(_includes(u.resolve()))):
             extra = ["-x", "hip"] if u.suffix == ".cpp" else []
             todo.append([HIPCC] + HIP_FLAGS + extra + ["-I", str(ROOT / "include"), "-c", str(u), "-o", str(o)])
     if todo:

@@ -199,9 +199,12 @@ __global__ void k_zero_int(int* p) {
 }
 
 // list[i] = i for i < batch, count = batch (every matrix takes the float64 solve)
-__global__ void k_iota(int* list, int* count, int64_t batch) {
+__global__ void k_iota(int* list, int* count, int* stat, int64_t batch) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < batch) list[i] = (int)i;
+  if (i < batch) {
+    list[i] = (int)i;
+    if (stat) stat[i] = -1;
+  }
   if (i == 0) *count = (int)batch;
 }
 

@@ -11,7 +11,7 @@ static hipError_t launch_ir(const IrArgs& a, int low, hipStream_t s) {
   if (low != 4) {
     hipLaunchKernelGGL((k_posv_ir<NB>), grid_for<NB>(a.batch), block_for<NB>(), 0, s, a, low);
   } else {  // float64 direct for every matrix: the list is 0..batch-1
-    hipLaunchKernelGGL(k_iota, dim3((unsigned)((a.batch + 255) / 256)), dim3(256), 0, s, a.flist, a.fcount, a.batch);
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((a.batch + 255) / 256)), dim3(256), 0, s, a.flist, a.fcount, a.stat, a.batch);
   }
   const dim3 g = grid_for<NB>(a.batch);
   const dim3 gg(g.x < 16384u ? g.x : 16384u);
