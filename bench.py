@@ -140,6 +140,8 @@ def main(argv=None):
     ap.add_argument("--factor", default="fp32", choices=["fp32", "bf16", "fp16", "fp64"],
                     help="low precision of the refinement tier")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 1 GiB device copy (reference kernel for scaling rocprof byte counters)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the batched solvers shard by "
@@ -221,6 +223,13 @@ def main(argv=None):
                    "mixed_over_plain": round(mixed_rate / plain_rate, 3)},
         "roofline": roofline,
     }
+    if a.copy_probe:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy probe: 3 x 1 GiB device copies done")
     if not a.no_cpu:
         log("[bench] CPU baseline (reference posv on the host cores)")
         out["cpu_baseline"] = cpu_baseline(A, B, nrhs)
