@@ -4,6 +4,7 @@
 
 #include "launch.hpp"
 #include "plain_kernels.hpp"
+#include "rows_kernels.hpp"
 
 namespace tsdx_hip {
 
@@ -19,7 +20,14 @@ inline dim3 block_for() {
 
 template <class T, int NB>
 hipError_t launch_potrf(T* A, int n, int64_t batch, int* info, int vec, hipStream_t s) {
-  hipLaunchKernelGGL((k_potrf<T, NB>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, info, vec);
+  using S = PotrfShape<T, NB>;
+  if constexpr (S::R == 1) {
+    hipLaunchKernelGGL((k_potrf<T, NB>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, info, vec);
+  } else {
+    constexpr int64_t per_block = (int64_t)S::WPB * (kWave / S::L);
+    const dim3 grid((unsigned)((batch + per_block - 1) / per_block));
+    hipLaunchKernelGGL((k_potrf_rows<T, S::L, S::R, S::WPB>), grid, dim3(kWave * S::WPB), 0, s, A, n, batch, info, vec);
+  }
   return hipGetLastError();
 }
 

@@ -1,0 +1,155 @@
+// Cholesky with several rows per lane ("L x R" groups).
+//
+// A matrix of size n <= N = L*R is owned by a group of L lanes; lane q holds
+// rows q, q+L, ..., q+(R-1)L (cyclic, so every lane keeps a share of the
+// long rows), slot s = row / L, column index static.  A wave therefore works
+// on 64/L matrices at once, and the per-column fixed cost (pivot broadcast,
+// square root, the LDS hand-off) is paid once for R rows per lane instead of
+// one.  Slots whose rows are all finished drop out of the update statically.
+//
+// The per-element operation chain is the reference's (kernels.hpp:18-37, the
+// same contract as plain_kernels.hpp): ascending-k fused updates, IEEE sqrt,
+// correctly rounded quotients.  For float the quotient a/d is formed as
+// RN_float(a * y) with y a Newton-refined float64 reciprocal of d: the float64
+// product is within 2^-52 of a/d, and a/d never lies closer than 2^-49
+// (relative) to a float rounding boundary unless it is one exactly, so the
+// rounding to float is the correctly rounded quotient.  The exceptions
+// (quotients in the float subnormal range, non-finite d) take IEEE division
+// for the whole wave.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace tsdx_hip {
+
+// Newton-refined reciprocal: v_rcp_f64 then two corrections.
+__device__ __forceinline__ double rcp_refined(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-d, y, 1.0);
+  return __builtin_fma(y, e, y);
+}
+
+// Correctly rounded a / d for every row slot of column k (see the header).
+template <class T, int R>
+struct ColumnDivider;
+
+template <int R>
+struct ColumnDivider<float, R> {
+  template <class Rows, class F>
+  __device__ __forceinline__ static void apply(Rows& r, float d, F&& store) {
+    const double y = rcp_refined((double)d);
+    float q[R];
+    bool slow = !(__builtin_fabsf(d) < __builtin_huge_valf());
+    static_for<0, R>([&](auto s) {
+      const double p = (double)r[s] * y;
+      const double ap = __builtin_fabs(p);
+      slow = slow || (ap != 0.0 && ap < 0x1p-125);
+      q[s] = (float)p;
+    });
+    if (__builtin_amdgcn_ballot_w64(slow) != 0) {
+      static_for<0, R>([&](auto s) { q[s] = r[s] / d; });
+    }
+    static_for<0, R>([&](auto s) { store(s, q[s]); });
+  }
+};
+
+template <int R>
+struct ColumnDivider<double, R> {
+  template <class Rows, class F>
+  __device__ __forceinline__ static void apply(Rows& r, double d, F&& store) {
+    static_for<0, R>([&](auto s) { store(s, r[s] / d); });
+  }
+};
+
+// Factor in registers.  r[s][j] = A(q + L*s, j) on entry, L on exit (entries
+// above the diagonal are garbage).  col: per-wave LDS, 2 * 64 * R elements.
+template <class T, int L, int R>
+__device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int n, int q, int g, T* col) {
+  constexpr int N = L * R;
+  int info = 0;
+  auto pivot = [&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    constexpr int kq = k % L, ks = k / L;
+    const T akk = bcast<L, kq>(r[ks][k]);
+    const bool bad = !(akk > T(0));
+    info = (bad && info == 0 && k < n) ? k + 1 : info;
+    const T d = sqrtT(bad ? T(1) : akk);
+    constexpr int S0 = ks;  // slots below ks hold rows < k only
+    T a[R - S0];
+    static_for<S0, R>([&](auto s) { a[s - S0] = r[s][k]; });
+    ColumnDivider<T, R - S0>::apply(a, d, [&](auto sl, T l) {
+      constexpr int s = decltype(sl)::value + S0;
+      const int row = q + L * s;
+      r[s][k] = (row == k) ? d : ((row > k) ? l : r[s][k]);
+    });
+  };
+  pivot(ic<0>{});
+  static_for<0, N - 1>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k + 1 < n) {
+      T* cb = col + (k & 1) * (kWave * R) + g * N;
+      static_for<k / L, R>([&](auto s) { cb[q + L * s] = r[s][k]; });
+      wave_sync();
+      constexpr int SA = (k + 1) / L;  // first slot with a row > k
+      static_for<SA, R>([&](auto s) { r[s][k + 1] = fmaT(-r[s][k], cb[k + 1], r[s][k + 1]); });
+      pivot(ic<k + 1>{});
+      static_for<k + 2, N>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const T c = cb[j];
+        static_for<SA, R>([&](auto s) { r[s][j] = fmaT(-r[s][k], c, r[s][j]); });
+      });
+    }
+  });
+  return info;
+}
+
+// Group shape per (type, bucket): L lanes x R rows, WPB waves per workgroup.
+// R = 1 selects the one-row kernel of plain_kernels.hpp.  Measured on MI355X
+// (uniform batches, fp32 n=32 65536 matrices: 301 us one-row, 171 us 8x4).
+template <class T, int NB>
+struct PotrfShape {
+  static constexpr int L = NB, R = 1, WPB = waves_per_block<NB>();
+};
+template <>
+struct PotrfShape<float, 16> {
+  static constexpr int L = 4, R = 4, WPB = 1;
+};
+template <>
+struct PotrfShape<float, 32> {
+  static constexpr int L = 8, R = 4, WPB = 1;
+};
+template <>
+struct PotrfShape<float, 64> {
+  static constexpr int L = 32, R = 2, WPB = 1;
+};
+template <>
+struct PotrfShape<double, 32> {
+  static constexpr int L = 8, R = 4, WPB = 1;
+};
+
+template <class T, int L, int R, int WPB>
+__global__ __launch_bounds__(kWave * WPB) void k_potrf_rows(T* __restrict__ A, int n, int64_t batch,
+                                                           int* __restrict__ info, int vec) {
+  constexpr int N = L * R;
+  __shared__ T col_s[WPB][2 * kWave * R];
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  const int g = lane / L, q = lane % L;
+  const int64_t m = ((int64_t)blockIdx.x * WPB + wv) * (kWave / L) + g;
+  const bool live = m < batch;
+  const int64_t nn = (int64_t)n * n;
+  T r[R][N];
+  static_for<0, R>([&](auto s) {
+    const int row = q + L * s;
+    load_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, live && row < n, vec != 0);
+  });
+  const int inf = chol_rows<T, L, R>(r, n, q, g, col_s[wv]);
+  static_for<0, R>([&](auto s) {
+    const int row = q + L * s;
+    if (live && row < n) store_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, vec != 0, true, row);
+  });
+  if (live && q == 0 && info) info[m] = inf;
+}
+
+}  // namespace tsdx_hip
