@@ -205,8 +205,8 @@ def main(argv=None):
     achieved = bytes_per * batch / mixed_kern / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"k_posv_ir<64> ({a.factor} factor + float64 refinement; the event window also holds the "
-                          "fallback-list reset and the gathered float64 launch)",
+                "kernel": f"k_posv_mx<4> ({a.factor} factor on MFMA + float64 refinement; the event window also "
+                          "holds the fallback-list reset and the gathered float64 launch)",
                 "bytes_per_matrix": bytes_per,
                 "plain_fp64_kernel": {"kernel": "k_posv<double,64,1>", "ms": round(plain_kern * 1e3, 4),
                                       "achieved": round(bytes_per * batch / plain_kern / 1e9, 1)}}

@@ -95,3 +95,28 @@ def test_ir_argument_errors(tsdx):
     assert L.tsdx_hip_posv_ir_f64(None, 65, 1, None, None, 1, 1, 10, 1e-14, None, None, None, 0, None) == _lib.EINVAL
     assert L.tsdx_hip_posv_ir_f64(None, 8, 1, None, None, 1, 9, 10, 1e-14, None, None, None, 0, None) == _lib.EINVAL
     assert L.tsdx_hip_posv_ir_f64(None, 8, 0, None, None, 1, 1, 10, 1e-14, None, None, None, 0, None) == _lib.OK
+
+
+@pytest.mark.parametrize("factor", ["fp32", "bf16"])
+def test_reads_lower_triangle_only(tsdx, factor):
+    """Like the plain posv (and the reference's Cholesky), only the lower
+    triangle of A is an input: garbage above the diagonal changes nothing."""
+    rng = np.random.default_rng(6)
+    A, B, _ = _system(rng, 32, 64, 1, 4, 2)
+    G = A.copy()
+    iu = np.triu_indices(64, 1)
+    G[:, iu[0], iu[1]] = 1e30
+    X, info, stat = _run(G, B, factor)
+    Xr, info_r, stat_r = _run(A, B, factor)
+    assert np.array_equal(X, Xr) and np.array_equal(stat, stat_r) and (info == 0).all()
+    assert _bwd(tsdx, A, X, B).max() <= 1e-14
+
+
+@pytest.mark.parametrize("n", [17, 33, 48, 63])
+def test_padded_sizes(tsdx, n):
+    """Sizes that leave part of the last 16 x 16 block as identity padding."""
+    rng = np.random.default_rng(n)
+    A, B, _ = _system(rng, 48, n, 1, 5, 3)
+    X, info, stat = _run(A, B, "fp32")
+    assert (info == 0).all() and (stat >= 0).mean() > 0.9, np.bincount(stat + 1)
+    assert _bwd(tsdx, A, X, B).max() <= 1e-14

@@ -157,6 +157,8 @@ void posv_ir(const double* A, int n, int64_t batch, const double* B, double* X, 
   if (factor < TSDX_HIP_FACTOR_AUTO || factor > TSDX_HIP_FACTOR_FP64) throw InvalidArg("unknown factor precision");
   if (max_iter < 0 || max_iter > 1000) throw InvalidArg("max_iter must be in [0, 1000]");
   if (!(tol > 0.0)) throw InvalidArg("tol must be positive");
+  // one workgroup per matrix and int entries in the fallback list
+  if (batch > 0x7fffffff) throw InvalidArg("batch above 2^31-1 is not supported by the refinement tier");
   if (batch == 0) return;
   if (!A || (nrhs > 0 && (!B || !X))) throw InvalidArg("A, B and X must not be null");
   if (!work || work_bytes < ir_workspace_size(batch)) throw InvalidArg("workspace missing or too small");

@@ -1,0 +1,355 @@
+// Mixed-precision posv with iterative refinement on the matrix cores (no
+// reference counterpart: the reference has only the plain drivers,
+// batched.hpp:271-275).
+//
+// One wave per matrix, n <= 16*NBLK.  The symmetric matrix is handled as
+// NBLK x NBLK blocks of 16 x 16, upper blocks only (I <= J), each block in the
+// accumulator layout of v_mfma_f32_16x16x4f32: lane l = 16g + j holds column j
+// and rows 4g + v (v = 0..3) of the block, four registers.  In that layout the
+// product X^T Y of two stored blocks is four MFMAs whose A / B operands are the
+// registers of X and Y as they stand, so the whole trailing update of a right-
+// looking block Cholesky (A = U^T U)
+//     U_KJ = U_KK^-T W_KJ,   W_IJ -= U_KI^T U_KJ      (K < I <= J)
+// runs on MFMA with no data movement.  The diagonal blocks are eliminated on
+// the VALU (16 steps each) and give the inverse V_KK = U_KK^-1 directly
+// (column operations on an identity), which is what the panel product and the
+// triangular solves use.
+//
+// Data kept per wave for the whole solve:
+//   ad   float64 upper blocks of A (80 VGPRs at NBLK = 4): the residual;
+//   w    float factor: V_KK in the diagonal slots, U_KJ elsewhere (40 VGPRs).
+// A is therefore read from HBM once.  Only the lower triangle of A is read
+// (the reference's convention for the Cholesky input): element (r, c) of the
+// symmetric matrix comes from A[max(r,c)][min(r,c)].  Rows and columns past n
+// are identity padding.
+//
+// Per right-hand side: x0 = U^-1 U^-T b (float), then up to max_iter steps of
+//   r = b - A x (float64), stop when ||r|| <= tol/2 (||A||_F ||x|| + ||b||),
+//   d = U^-1 U^-T r (float), x += d (float64).
+// Matrices whose factorization meets a non-positive pivot, or whose
+// refinement stalls or runs out of steps, go to the float64 fallback list.
+//
+// Vector forms used by the solves (values of a length-16*NBLK vector):
+//   column form  c[J]    lane (g, j) holds entry 16J + j (same for every g);
+//   row form     R[I][v()] lane (g, j) holds entry 16I + 4g + v (same for every j).
+#pragma once
+
+#include "mixed_kernels.hpp"
+
+namespace tsdx_hip {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int NBLK>
+constexpr int nupper() {
+  return NBLK * (NBLK + 1) / 2;
+}
+// Slot of upper block (I, J), I <= J, blocks stored row by row.
+template <int NBLK>
+constexpr int bslot(int I, int J) {
+  return I * NBLK - I * (I - 1) / 2 + (J - I);
+}
+
+// ------------------------------------------------------------ lane moves
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dppf(float x) {
+  return __builtin_bit_cast(float, dpp_i32<CTRL>(__builtin_bit_cast(int, x)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dppd(double x) {
+  return mk64(dpp_i32<CTRL>(lo32(x)), dpp_i32<CTRL>(hi32(x)));
+}
+// DPP controls (rows of 16 lanes): row_newbcast:K = 0x150 + K (every lane of
+// the row reads lane K of the row), row_ror:8 = 0x128 (partner j ^ 8),
+// row_half_mirror = 0x141 (partner j ^ 7), quad_perm xor 2 = 0x4E, xor 1 = 0xB1.
+template <int K>
+__device__ __forceinline__ float row_bcast(float x) {
+  return dppf<0x150 + K>(x);
+}
+__device__ __forceinline__ float bperm(float x, int src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src << 2, __builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double bperm(double x, int src) {
+  return mk64(__builtin_amdgcn_ds_bpermute(src << 2, lo32(x)), __builtin_amdgcn_ds_bpermute(src << 2, hi32(x)));
+}
+// Sum over the four lanes that share j (the row groups): xor 16, xor 32.
+template <class V>
+__device__ __forceinline__ V sum_g(V x) {
+  x += xshfl<16>(x);
+  x += xshfl<32>(x);
+  return x;
+}
+// Sum over the 16 lanes of a row group.
+template <class V>
+__device__ __forceinline__ V sum_j(V x) {
+  x += xshfl<1>(x);
+  x += xshfl<2>(x);
+  x += xshfl<7>(x);
+  x += xshfl<15>(x);
+  return x;
+}
+
+// Transposed sum over the 16 lanes of a row group: on return lane j holds the
+// row-group sum of p[j >> 2].  Butterfly steps with partners j^8 and j^7 (a
+// lane keeps the half of its values selected by bit 3, then bit 2, of j and
+// receives the same half from its partner), then j^2 and j^1 on the single
+// value left; the masks 8, 7, 2, 1 are independent, so every lane of the row
+// group is summed once.
+// (Operands go through an empty asm first: a select between two elements of a
+// private array would otherwise become a load from a computed address, which
+// keeps the array out of registers.)
+template <class V>
+__device__ __forceinline__ void in_vgpr(V& x) {
+  asm volatile("" : "+v"(x));
+}
+template <class V>
+__device__ __forceinline__ V tsum4(const V (&p)[4], int j) {
+  const bool b3 = j & 8, b2 = j & 4;
+  V q[4] = {p[0], p[1], p[2], p[3]};
+  static_for<0, 4>([&](auto t) { in_vgpr(q[t]); });
+  V a[2];
+  static_for<0, 2>([&](auto t) {
+    const V send = b3 ? q[t] : q[t + 2], keep = b3 ? q[t + 2] : q[t];
+    if constexpr (sizeof(V) == 8) a[t] = keep + dppd<0x128>(send); else a[t] = keep + dppf<0x128>(send);
+  });
+  in_vgpr(a[0]);
+  in_vgpr(a[1]);
+  const V send = b2 ? a[0] : a[1], keep = b2 ? a[1] : a[0];
+  V c;
+  if constexpr (sizeof(V) == 8) {
+    c = keep + dppd<0x141>(send);
+    c += dppd<0x4E>(c);
+    c += dppd<0xB1>(c);
+  } else {
+    c = keep + dppf<0x141>(send);
+    c += dppf<0x4E>(c);
+    c += dppf<0xB1>(c);
+  }
+  return c;
+}
+
+// ------------------------------------------------- diagonal block elimination
+// W: symmetric 16 x 16 block (accumulator layout).  Returns V = U^-1 where
+// W = U^T U (U upper, entries rounded to the low format), and sets bad when a
+// pivot is not positive (the pivot is then taken as 1).
+template <Low LF>
+__device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
+  // fresh lane indices per call: the 16 x 5 lane masks of the steps are then
+  // formed where they are used instead of being kept live across calls
+  g = opaque(g);
+  j = opaque(j);
+  v4f F;
+  static_for<0, 4>([&](auto v) { F[v()] = (4 * g + v == j) ? 1.0f : 0.0f; });
+  const int src_row_base = j;  // + 16 * (k / 4): lane holding row k, column j
+  static_for<0, 16>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    constexpr int kg = k / 4, kv = k % 4;
+    // (the element is copied to a scalar first: __builtin_bit_cast applied to
+    // an element of a vector type reads element 0)
+    const float wkv = W[kv];
+    const float akk = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wkv), 16 * kg + k));
+    const bool nb = !(akk > 0.0f);
+    bad = bad || nb;
+    const float rd = __builtin_amdgcn_rsqf(nb ? 1.0f : akk);
+    // u_j = U[k][j] (this lane's column), u_i[v()] = U[k][4g+v] = W[4g+v][k] * rd
+    float uj = round_low(bperm(W[kv], 16 * kg + src_row_base) * rd, LF);
+    uj = (j > k) ? uj : 0.0f;
+    float fk[4];
+    static_for<0, 4>([&](auto v) {
+      float ui = round_low(row_bcast<k>(W[v()]) * rd, LF);
+      ui = (4 * g + v > k) ? ui : 0.0f;
+      W[v()] = __builtin_fmaf(-ui, uj, W[v()]);
+      fk[v()] = row_bcast<k>(F[v()]) * rd;
+    });
+    static_for<0, 4>([&](auto v) { F[v()] = (j == k) ? fk[v()] : __builtin_fmaf(-uj, fk[v()], F[v()]); });
+  });
+  return F;
+}
+
+// ------------------------------------------------------------------ kernel
+template <int NBLK, Low LF>
+__global__ __launch_bounds__(kWave) void k_posv_mx(IrArgs a) {
+  constexpr int NU = nupper<NBLK>();
+  constexpr int NP = 16 * NBLK;
+  __shared__ __align__(16) double xs[NP];
+  __shared__ __align__(16) float fs[16];
+  const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
+  const int64_t m = blockIdx.x;
+  const int n = a.n;
+  const double* Am = a.A + m * (int64_t)n * n;
+
+  // ---- load the upper blocks of the symmetric matrix (from the lower triangle)
+  double ad[NU][4];
+  double ss = 0.0;
+  static_for<0, NBLK>([&](auto Ic) {
+    static_for<Ic, NBLK>([&](auto Jc) {
+      constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
+      const int c = 16 * J + j;
+      static_for<0, 4>([&](auto v) {
+        const int r = 16 * I + 4 * g + v;
+        double x;
+        if (r < n && c < n) {
+          const int hi = r > c ? r : c, lo = r > c ? c : r;
+          x = Am[(int64_t)hi * n + lo];
+          ss = __builtin_fma(x, (I == J) ? x : 2.0 * x, ss);
+        } else {
+          x = (r == c) ? 1.0 : 0.0;
+        }
+        ad[b][v()] = x;
+      });
+    });
+  });
+  const double anorm = __builtin_sqrt(sum_j(sum_g(ss)));
+
+  // ---- factor: float copy, block right-looking, MFMA trailing updates
+  v4f w[NU];
+  static_for<0, NU>([&](auto b) {
+    static_for<0, 4>([&](auto v) { w[b][v()] = round_low((float)ad[b][v()], LF); });
+  });
+  bool bad = false;
+  static_for<0, NBLK>([&](auto Kc) {
+    constexpr int K = decltype(Kc)::value;
+    const v4f V = diag_inverse<LF>(w[bslot<NBLK>(K, K)], g, j, bad);
+    w[bslot<NBLK>(K, K)] = V;
+    static_for<K + 1, NBLK>([&](auto Jc) {
+      constexpr int J = decltype(Jc)::value;
+      v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      static_for<0, 4>([&](auto v) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(V[v()], w[bslot<NBLK>(K, J)][v()], acc, 0, 0, 0);
+      });
+      static_for<0, 4>([&](auto v) { acc[v()] = round_low(acc[v()], LF); });
+      w[bslot<NBLK>(K, J)] = acc;
+    });
+    static_for<K + 1, NBLK>([&](auto Ic) {
+      static_for<Ic, NBLK>([&](auto Jc) {
+        constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
+        v4f acc = w[bslot<NBLK>(I, J)];
+        static_for<0, 4>([&](auto v) {
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-w[bslot<NBLK>(K, I)][v()], w[bslot<NBLK>(K, J)][v()], acc, 0, 0, 0);
+        });
+        w[bslot<NBLK>(I, J)] = acc;
+      });
+    });
+  });
+
+  // ---- solves with the float factor: t (column form) -> z (column form)
+  // one block, column form -> row form, through LDS
+  auto c2r = [&](float c, float (&R)[4]) __attribute__((always_inline)) {
+    if (g == 0) fs[j] = c;
+    wave_sync();
+    const float4 q = *reinterpret_cast<const float4*>(fs + 4 * g);
+    R[0] = q.x; R[1] = q.y; R[2] = q.z; R[3] = q.w;
+    wave_sync();
+  };
+  // source lane of entry j (column form) in the "lane j holds row 4g + (j >> 2)" form
+  const int src_mixed = 16 * (j >> 2) + 4 * (j & 3);
+  auto solve = [&](const float (&t)[NBLK], float (&z)[NBLK]) __attribute__((always_inline)) {
+    float yc[NBLK], yR[NBLK][4];
+    static_for<0, NBLK>([&](auto Kc) {
+      constexpr int K = decltype(Kc)::value;
+      float acc = 0.0f;
+      static_for<0, K>([&](auto I) {
+        static_for<0, 4>([&](auto v) { acc = __builtin_fmaf(w[bslot<NBLK>(I, K)][v()], yR[I][v()], acc); });
+      });
+      float sR[4];
+      c2r(t[K] - sum_g(acc), sR);
+      float p = 0.0f;
+      static_for<0, 4>([&](auto v) { p = __builtin_fmaf(w[bslot<NBLK>(K, K)][v()], sR[v()], p); });
+      yc[K] = sum_g(p);
+      c2r(yc[K], yR[K]);
+    });
+    static_for_rev<0, NBLK>([&](auto Kc) {
+      constexpr int K = decltype(Kc)::value;
+      float p[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      static_for<K + 1, NBLK>([&](auto J) {
+        static_for<0, 4>([&](auto v) { p[v()] = __builtin_fmaf(w[bslot<NBLK>(K, J)][v()], z[J], p[v()]); });
+      });
+      const float wc = yc[K] - bperm(tsum4(p, j), src_mixed);  // column form
+      float q[4];
+      static_for<0, 4>([&](auto v) { q[v()] = w[bslot<NBLK>(K, K)][v()] * wc; });
+      z[K] = bperm(tsum4(q, j), src_mixed);
+    });
+  };
+
+  bool ok = !bad;
+  int used = 0;
+  const double* Bm = a.B + m * (int64_t)n * a.nrhs;
+  double* Xm = a.X + m * (int64_t)n * a.nrhs;
+  for (int c = 0; c < a.nrhs && ok; ++c) {
+    double bc[NBLK], xc[NBLK];
+    float t[NBLK], z[NBLK];
+    double bb = 0.0;
+    static_for<0, NBLK>([&](auto J) {
+      const int row = 16 * J + j;
+      bc[J] = row < n ? Bm[(int64_t)row * a.nrhs + c] : 0.0;
+      bb = __builtin_fma(bc[J], bc[J], bb);
+      t[J] = (float)bc[J];
+    });
+    const double bnorm = __builtin_sqrt(sum_j(bb));
+    solve(t, z);
+    static_for<0, NBLK>([&](auto J) { xc[J] = (16 * J + j < n) ? (double)z[J] : 0.0; });
+    bool conv = false;
+    double prev = __builtin_huge_val();
+    for (int it = 0; it <= a.max_iter; ++it) {
+      // r = b - A x in float64 with the stored upper blocks
+      if (g == 0) static_for<0, NBLK>([&](auto J) { xs[16 * J + j] = xc[J]; });
+      wave_sync();
+      double pc[NBLK], rc[NBLK];
+      static_for<0, NBLK>([&](auto J) { pc[J] = 0.0; });
+      static_for<0, NBLK>([&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        const double2 q0 = *reinterpret_cast<const double2*>(xs + 16 * I + 4 * g);
+        const double2 q1 = *reinterpret_cast<const double2*>(xs + 16 * I + 4 * g + 2);
+        const double xR[4] = {q0.x, q0.y, q1.x, q1.y};  // row form of block I
+        double p[4] = {0.0, 0.0, 0.0, 0.0};
+        static_for<I, NBLK>([&](auto Jc) {
+          constexpr int J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
+          static_for<0, 4>([&](auto v) {
+            p[v] = __builtin_fma(ad[b][v()], xc[J], p[v]);
+            if constexpr (I < J) pc[J] = __builtin_fma(ad[b][v()], xR[v], pc[J]);
+          });
+        });
+        rc[I] = bperm(tsum4(p, j), src_mixed);  // rows of block I through upper blocks, column form
+      });
+      wave_sync();
+      double rr = 0.0, xx = 0.0;
+      static_for<0, NBLK>([&](auto J) {
+        rc[J] = (16 * J + j < n) ? bc[J] - rc[J] - sum_g(pc[J]) : 0.0;
+        rr = __builtin_fma(rc[J], rc[J], rr);
+        xx = __builtin_fma(xc[J], xc[J], xx);
+      });
+      const double rn = __builtin_sqrt(sum_j(rr)), xn = __builtin_sqrt(sum_j(xx));
+      if (rn <= 0.5 * a.tol * (anorm * xn + bnorm)) {
+        conv = true;
+        used = used > it ? used : it;
+        break;
+      }
+      if (it == a.max_iter || !(rn < 0.5 * prev)) break;
+      prev = rn;
+      static_for<0, NBLK>([&](auto J) { t[J] = (float)rc[J]; });
+      solve(t, z);
+      static_for<0, NBLK>([&](auto J) { xc[J] += (16 * J + j < n) ? (double)z[J] : 0.0; });
+    }
+    ok = conv;
+    if (conv && g == 0) {
+      static_for<0, NBLK>([&](auto J) {
+        const int row = 16 * J + j;
+        if (row < n) Xm[(int64_t)row * a.nrhs + c] = xc[J];
+      });
+    }
+  }
+  if (lane == 0) {
+    if (!ok) {
+      const int slot = atomicAdd(a.fcount, 1);
+      a.flist[slot] = (int)m;
+    }
+    if (a.info) a.info[m] = 0;
+    if (a.stat) a.stat[m] = ok ? used : -1;
+  }
+}
+
+}  // namespace tsdx_hip

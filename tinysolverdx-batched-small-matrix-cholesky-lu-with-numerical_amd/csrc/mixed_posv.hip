@@ -1,15 +1,27 @@
 // Launchers of the mixed-precision refinement tier (mixed_kernels.hpp).
 #include "capi_internal.hpp"
 #include "launch_impl.hpp"
-#include "mixed_kernels.hpp"
+#include "mfma_kernels.hpp"
 
 namespace tsdx_hip {
+
+// Matrix-core refinement kernel: one wave (one workgroup) per matrix.
+template <int NBLK>
+static void launch_mx(const IrArgs& a, int low, hipStream_t s) {
+  const dim3 grid((unsigned)a.batch), block(kWave);
+  if (low == 2)
+    hipLaunchKernelGGL((k_posv_mx<NBLK, Low::BF16>), grid, block, 0, s, a);
+  else if (low == 3)
+    hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F16>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F32>), grid, block, 0, s, a);
+}
 
 template <int NB>
 static hipError_t launch_ir(const IrArgs& a, int low, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, a.fcount);
   if (low != 4) {
-    hipLaunchKernelGGL((k_posv_ir<NB>), grid_for<NB>(a.batch), block_for<NB>(), 0, s, a, low);
+    launch_mx<(NB + 15) / 16>(a, low, s);
   } else {  // float64 direct for every matrix: the list is 0..batch-1
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((a.batch + 255) / 256)), dim3(256), 0, s, a.flist, a.fcount, a.stat, a.batch);
   }
