@@ -51,9 +51,12 @@ constexpr int bslot(int I, int J) {
 }
 
 // ------------------------------------------------------------ lane moves
+// Every control used here reads a valid lane for every lane, so the move
+// needs no "old" value (bound_ctrl set), which lets the compiler fold it into
+// the consuming VALU operation.
 template <int CTRL>
 __device__ __forceinline__ int dpp_i32(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ float dppf(float x) {
@@ -76,12 +79,32 @@ __device__ __forceinline__ float bperm(float x, int src) {
 __device__ __forceinline__ double bperm(double x, int src) {
   return mk64(__builtin_amdgcn_ds_bpermute(src << 2, lo32(x)), __builtin_amdgcn_ds_bpermute(src << 2, hi32(x)));
 }
-// Sum over the four lanes that share j (the row groups): xor 16, xor 32.
-template <class V>
-__device__ __forceinline__ V sum_g(V x) {
-  x += xshfl<16>(x);
-  x += xshfl<32>(x);
-  return x;
+// Sum over the four lanes that share j (the row groups), on the VALU:
+// v_permlane16_swap(x, x) returns (x with rows 1 and 3 replaced by rows 0 and
+// 2, x with rows 0 and 2 replaced by rows 1 and 3), so the sum of the pair is
+// the sum over lanes l and l ^ 16; v_permlane32_swap does the same for halves.
+__device__ __forceinline__ float sum_g(float x) {
+  const unsigned u = __builtin_bit_cast(unsigned, x);
+  const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+  x = __builtin_bit_cast(float, (unsigned)a[0]) + __builtin_bit_cast(float, (unsigned)a[1]);
+  const unsigned w = __builtin_bit_cast(unsigned, x);
+  const auto b = __builtin_amdgcn_permlane32_swap(w, w, false, false);
+  return __builtin_bit_cast(float, (unsigned)b[0]) + __builtin_bit_cast(float, (unsigned)b[1]);
+}
+__device__ __forceinline__ double sum_g(double x) {
+  auto swap16 = [](double y) {
+    const unsigned lo = (unsigned)lo32(y), hi = (unsigned)hi32(y);
+    const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return mk64((int)l[0], (int)h[0]) + mk64((int)l[1], (int)h[1]);
+  };
+  auto swap32 = [](double y) {
+    const unsigned lo = (unsigned)lo32(y), hi = (unsigned)hi32(y);
+    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return mk64((int)l[0], (int)h[0]) + mk64((int)l[1], (int)h[1]);
+  };
+  return swap32(swap16(x));
 }
 // Sum over the 16 lanes of a row group.
 template <class V>
@@ -136,15 +159,26 @@ __device__ __forceinline__ V tsum4(const V (&p)[4], int j) {
 // W: symmetric 16 x 16 block (accumulator layout).  Returns V = U^-1 where
 // W = U^T U (U upper, entries rounded to the low format), and sets bad when a
 // pivot is not positive (the pivot is then taken as 1).
+//
+// With U = D T (D = diag(U), T unit upper, T[k][j] = W_k[k][j] / W_k[k][k]
+// for the partly reduced W_k), step k does
+//     W[i][j] -= W[i][k] T[k][j]              (i, j > k: the Schur update)
+//     F[:,j]  -= T[k][j] F[:,k]   (j > k)     (F = identity at the start)
+// which leaves F = T^-1; V = T^-1 D^-1 scales column j by 1/U[j][j] at the end.
+// The update of W is applied to every entry: rows and columns <= k then hold
+// garbage, but W is only read afterwards at row k', column k' and the
+// diagonal (k' > k) of the live part, and the garbage only ever reaches rows
+// or columns <= k'; W itself is dropped at the end.  Only the update of F
+// needs T[k][j] masked to j > k.
 template <Low LF>
 __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
-  // fresh lane indices per call: the 16 x 5 lane masks of the steps are then
-  // formed where they are used instead of being kept live across calls
+  // fresh lane indices per call: the lane masks of the steps are formed where
+  // they are used instead of being kept live across calls
   g = opaque(g);
   j = opaque(j);
   v4f F;
   static_for<0, 4>([&](auto v) { F[v()] = (4 * g + v == j) ? 1.0f : 0.0f; });
-  const int src_row_base = j;  // + 16 * (k / 4): lane holding row k, column j
+  float rdl = 1.0f;  // 1 / U[j][j] of this lane's column
   static_for<0, 16>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     constexpr int kg = k / 4, kv = k % 4;
@@ -155,18 +189,24 @@ __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
     const bool nb = !(akk > 0.0f);
     bad = bad || nb;
     const float rd = __builtin_amdgcn_rsqf(nb ? 1.0f : akk);
-    // u_j = U[k][j] (this lane's column), u_i[v()] = U[k][4g+v] = W[4g+v][k] * rd
-    float uj = round_low(bperm(W[kv], 16 * kg + src_row_base) * rd, LF);
-    uj = (j > k) ? uj : 0.0f;
-    float fk[4];
-    static_for<0, 4>([&](auto v) {
-      float ui = round_low(row_bcast<k>(W[v()]) * rd, LF);
-      ui = (4 * g + v > k) ? ui : 0.0f;
-      W[v()] = __builtin_fmaf(-ui, uj, W[v()]);
-      fk[v()] = row_bcast<k>(F[v()]) * rd;
-    });
-    static_for<0, 4>([&](auto v) { F[v()] = (j == k) ? fk[v()] : __builtin_fmaf(-uj, fk[v()], F[v()]); });
+    const float wk = bperm(wkv, 16 * kg + j);  // W[k][j], from row k (lane 16 kg + j)
+    float t;                                   // T[k][j]
+    if constexpr (LF == Low::F32) {
+      t = wk * (rd * rd);
+      static_for<0, 4>([&](auto v) { W[v()] = __builtin_fmaf(-row_bcast<k>(W[v()]), t, W[v()]); });
+    } else {  // U entries rounded to the low format
+      const float uj = round_low(wk * rd, LF);
+      t = uj * rd;
+      static_for<0, 4>([&](auto v) {
+        const float ui = round_low(row_bcast<k>(W[v()]) * rd, LF);
+        W[v()] = __builtin_fmaf(-ui, uj, W[v()]);
+      });
+    }
+    const float tm = (j > k) ? t : 0.0f;
+    rdl = (j == k) ? rd : rdl;
+    static_for<0, 4>([&](auto v) { F[v()] = __builtin_fmaf(-tm, row_bcast<k>(F[v()]), F[v()]); });
   });
+  static_for<0, 4>([&](auto v) { F[v()] *= rdl; });
   return F;
 }
 
@@ -183,24 +223,41 @@ __global__ __launch_bounds__(kWave) void k_posv_mx(IrArgs a) {
   const double* Am = a.A + m * (int64_t)n * n;
 
   // ---- load the upper blocks of the symmetric matrix (from the lower triangle)
+  // Off-diagonal block (I < J): lane (g, j) holds rows 16I + 4g + v, column
+  // 16J + j, i.e. A[16J + j][16I + 4g .. 4g + 3] (four consecutive entries of
+  // a lower row: two 16-byte loads when rows are 32-byte aligned, vec).
+  // Diagonal blocks read A[max(r,c)][min(r,c)] entry by entry.
+  const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15) == 0);
   double ad[NU][4];
   double ss = 0.0;
   static_for<0, NBLK>([&](auto Ic) {
     static_for<Ic, NBLK>([&](auto Jc) {
       constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
       const int c = 16 * J + j;
-      static_for<0, 4>([&](auto v) {
-        const int r = 16 * I + 4 * g + v;
-        double x;
-        if (r < n && c < n) {
-          const int hi = r > c ? r : c, lo = r > c ? c : r;
-          x = Am[(int64_t)hi * n + lo];
-          ss = __builtin_fma(x, (I == J) ? x : 2.0 * x, ss);
-        } else {
-          x = (r == c) ? 1.0 : 0.0;
+      if (I < J && vec) {
+        const int r0 = 16 * I + 4 * g;
+        double2 q0 = {0.0, 0.0}, q1 = {0.0, 0.0};  // past n: identity padding, off the diagonal
+        if (r0 < n && c < n) {
+          const double2* p = reinterpret_cast<const double2*>(Am + (int64_t)c * n + r0);
+          q0 = p[0];
+          q1 = p[1];
         }
-        ad[b][v()] = x;
-      });
+        ad[b][0] = q0.x; ad[b][1] = q0.y; ad[b][2] = q1.x; ad[b][3] = q1.y;
+        static_for<0, 4>([&](auto v) { ss = __builtin_fma(ad[b][v()], 2.0 * ad[b][v()], ss); });
+      } else {
+        static_for<0, 4>([&](auto v) {
+          const int r = 16 * I + 4 * g + v;
+          double x;
+          if (r < n && c < n) {
+            const int hi = r > c ? r : c, lo = r > c ? c : r;
+            x = Am[(int64_t)hi * n + lo];
+            ss = __builtin_fma(x, (I == J) ? x : 2.0 * x, ss);
+          } else {
+            x = (r == c) ? 1.0 : 0.0;
+          }
+          ad[b][v()] = x;
+        });
+      }
     });
   });
   const double anorm = __builtin_sqrt(sum_j(sum_g(ss)));
