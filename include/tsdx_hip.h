@@ -123,6 +123,42 @@ int tsdx_hip_posv_ir_f64(const double *A, int n, int64_t batch, const double *B,
                          int max_iter, double tol, int *info, int *stat, void *work, size_t work_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Ragged batches: matrices of different sizes (1..64) and kinds in one call.
+ * No reference counterpart (the reference drivers take one n per call,
+ * batched.hpp:121-244); each matrix gets exactly the result of the per-size
+ * call of its kind (tsdx_hip_potrf / tsdx_hip_getrf), bit for bit.
+ *   A          device buffer; matrix m (row-major n_m x n_m) starts at element
+ *              offsets[m]; factored in place.
+ *   sizes      (count) n_m;  kinds (count) TSDX_HIP_KIND_CHOLESKY or _LU.
+ *   piv        pivots of LU matrix m at element piv_offsets[m] (n_m ints);
+ *              may be NULL (then piv_offsets may be NULL).
+ *   info       (count) LAPACK codes, may be NULL.
+ * A plan reads sizes and kinds once (synchronising `stream`), buckets the
+ * matrices by kind and size, and, when tune != 0, times the compiled variants
+ * of each size class on synthetic matrices of the same sizes (the caller's
+ * data is not touched) and keeps the fastest.  It captures the work - one
+ * persistent kernel per size class fed by a work queue - into a HIP graph;
+ * tsdx_hip_ragged_run replays it on `stream` (one graph launch).  The plan is
+ * bound to the buffers it was created with; runs of one plan must not overlap.
+ * ------------------------------------------------------------------------ */
+#define TSDX_HIP_KIND_CHOLESKY 0
+#define TSDX_HIP_KIND_LU 1
+
+typedef struct tsdx_hip_ragged_plan tsdx_hip_ragged_plan;
+
+int tsdx_hip_ragged_plan_create_f32(float *A, const int64_t *offsets, const int *sizes, const int *kinds, int64_t count,
+                                    int *piv, const int64_t *piv_offsets, int *info, int tune,
+                                    tsdx_hip_ragged_plan **plan, void *stream);
+int tsdx_hip_ragged_plan_create_f64(double *A, const int64_t *offsets, const int *sizes, const int *kinds,
+                                    int64_t count, int *piv, const int64_t *piv_offsets, int *info, int tune,
+                                    tsdx_hip_ragged_plan **plan, void *stream);
+int tsdx_hip_ragged_run(tsdx_hip_ragged_plan *plan, void *stream);
+/* JSON text: per size class the matrices, wave tasks, chosen variant, grid and
+ * the tuner's timings. */
+const char *tsdx_hip_ragged_plan_report(const tsdx_hip_ragged_plan *plan);
+int tsdx_hip_ragged_plan_destroy(tsdx_hip_ragged_plan *plan);
+
+/* ---------------------------------------------------------------------------
  * Host-pointer entry points (synchronous).  Shapes of the reference's CUDA
  * backend (cuda_backend.hpp:10-11) plus the solve and driver calls of
  * batched.hpp:248-280.

@@ -138,3 +138,18 @@ def test_matrix_generators_have_prescribed_condition():
     np.testing.assert_allclose(np.linalg.cond(S), 1e4, rtol=1e-6)
     G = matrices.general_kappas(rng, np.array([10.0, 1e3]), 9)
     np.testing.assert_allclose(np.linalg.cond(G), [10.0, 1e3], rtol=1e-6)
+
+
+def test_ragged_plan_argument_errors(tsdx):
+    """Checks that need no device: null buffers, negative count, null plan."""
+    import ctypes as C
+    from tinysolverdx import _lib
+    L = _lib.lib()
+    plan = C.c_void_p()
+    assert L.tsdx_hip_ragged_plan_create_f32(None, None, None, None, 3, None, None, None, 1, C.byref(plan), None) == _lib.EINVAL
+    assert "null" in _lib.last_error()
+    assert L.tsdx_hip_ragged_plan_create_f64(None, None, None, None, -1, None, None, None, 1, C.byref(plan), None) == _lib.EINVAL
+    assert L.tsdx_hip_ragged_plan_create_f32(None, None, None, None, 0, None, None, None, 1, None, None) == _lib.EINVAL
+    assert L.tsdx_hip_ragged_run(None, None) == _lib.EINVAL
+    assert L.tsdx_hip_ragged_plan_report(None) == b""
+    assert L.tsdx_hip_ragged_plan_destroy(None) == _lib.OK

@@ -1,0 +1,3 @@
+// Instantiates the ragged-batch persistent kernel for float, register class n <= 32, variant 1.
+#include "ragged_launch_impl.hpp"
+TSDX_INSTANTIATE_RAGGED(float, 1, 1)

@@ -1,0 +1,3 @@
+// Instantiates the ragged-batch persistent kernel for double, register class n <= 16, variant 1.
+#include "ragged_launch_impl.hpp"
+TSDX_INSTANTIATE_RAGGED(double, 0, 1)

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_posv_gather(I
     const bool io = live && i < n;
     double r[NB];
     load_row<double, NB>(r, a.A + m * nn + (int64_t)i * n, n, io, false);
-    const int inf = chol_regs<double, NB>(r, n, i, g, col_s[wv]);
+    const int inf = chol_regs<double, NB>(r, n, n, i, g, col_s[wv]);
     if (live && i == 0 && a.info) a.info[m] = inf;
     const double dinv = diag_inv<double, NB>(r, i);
     for (int c0 = 0; c0 < a.nrhs; c0 += NR) {

@@ -38,8 +38,11 @@ namespace tsdx_hip {
 // pivot work of column k+1 (broadcast of the diagonal, sqrt, the division of
 // the column) right after it, so that dependent chain overlaps the remaining
 // independent updates of step k instead of following them.
+// nloop: wave-uniform step bound (the largest size among the wave's groups);
+// n: this group's size (info only counts steps k < n; steps past a group's
+// size touch only its zero padding).
 template <class T, int NB>
-__device__ __forceinline__ int chol_regs(T (&r)[NB], int n, int i, int g, T* col) {
+__device__ __forceinline__ int chol_regs(T (&r)[NB], int nloop, int n, int i, int g, T* col) {
   int info = 0;
   auto pivot = [&](auto kc) {
     constexpr int k = decltype(kc)::value;
@@ -53,7 +56,7 @@ __device__ __forceinline__ int chol_regs(T (&r)[NB], int n, int i, int g, T* col
   pivot(ic<0>{});
   static_for<0, NB - 1>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    if (k + 1 < n) {
+    if (k + 1 < nloop) {
       T* cb = col + (k & 1) * kWave + g * NB;
       cb[i] = r[k];
       wave_sync();
@@ -81,15 +84,17 @@ __device__ __forceinline__ T diag_inv(const T (&r)[NB], int i) {
 // Rows never move between lanes: lane i keeps original row i and tracks its
 // current position `pos`.  On exit row `pos` of the packed L\U factor is in r,
 // lane k holds piv[k] in mypiv.  rowb: per-wave LDS, 2*64 elements.
+// nloop / n as for chol_regs.
 template <class T, int NB, bool ZTILE>
-__device__ __forceinline__ void lu_regs(T (&r)[NB], int n, int i, int g, T* rowb, int& pos, int& mypiv, int& info) {
+__device__ __forceinline__ void lu_regs(T (&r)[NB], int nloop, int n, int i, int g, T* rowb, int& pos, int& mypiv,
+                                        int& info) {
   pos = i;
   mypiv = 0;
   info = 0;
   const bool act = i < n;
   static_for<0, NB>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    if (k < n) {
+    if (k < nloop) {
       const bool cand = act && pos >= k;
       const T v = absT(r[k]);
       T key = (v != v) ? (pos == k ? infT<T>() : T(-1)) : v;
@@ -107,7 +112,7 @@ __device__ __forceinline__ void lu_regs(T (&r)[NB], int n, int i, int g, T* rowb
       wave_sync();
       const T pk = rb[k];
       const bool zero = (pk == T(0));
-      info = (zero && info == 0) ? k + 1 : info;
+      info = (zero && info == 0 && k < n) ? k + 1 : info;
       if (act && pos > k && (ZTILE || !zero)) {
         const T f = r[k] / (zero ? T(1) : pk);
         r[k] = f;
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_potrf(T* __re
   const bool io = t.live && t.i < n;
   T r[NB];
   load_row<T, NB>(r, row, n, io, vec != 0);
-  const int inf = chol_regs<T, NB>(r, n, t.i, t.g, col_s[wv]);
+  const int inf = chol_regs<T, NB>(r, n, n, t.i, t.g, col_s[wv]);
   if (io) store_row<T, NB>(r, row, n, vec != 0, true, t.i);
   if (t.live && t.i == 0 && info) info[t.m] = inf;
 }
@@ -277,7 +282,7 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_getrf(T* __re
   T r[NB];
   load_row<T, NB>(r, base + (int64_t)t.i * n, n, io, vec != 0);
   int pos, mypiv, inf;
-  lu_regs<T, NB, ZTILE>(r, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
+  lu_regs<T, NB, ZTILE>(r, n, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
   if (io) {
     store_row<T, NB>(r, base + (int64_t)pos * n, n, vec != 0, false, 0);
     piv[t.m * n + t.i] = mypiv;
@@ -360,7 +365,7 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_posv(const T*
   const bool io = t.live && t.i < n;
   T r[NB];
   load_row<T, NB>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
-  const int inf = chol_regs<T, NB>(r, n, t.i, t.g, col_s[wv]);
+  const int inf = chol_regs<T, NB>(r, n, n, t.i, t.g, col_s[wv]);
   if (Lout && io) store_row<T, NB>(r, Lout + t.m * nn + (int64_t)t.i * n, n, vec != 0, true, t.i);
   if (t.live && t.i == 0 && info) info[t.m] = inf;
   const T dinv = diag_inv<T, NB>(r, t.i);
@@ -388,7 +393,7 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_gesv(const T*
   T r[NB];
   load_row<T, NB>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
   int pos, mypiv, inf;
-  lu_regs<T, NB, ZTILE>(r, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
+  lu_regs<T, NB, ZTILE>(r, n, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
   if (io) {
     if (LUout) store_row<T, NB>(r, LUout + t.m * nn + (int64_t)pos * n, n, vec != 0, false, 0);
     if (piv) piv[t.m * n + t.i] = mypiv;

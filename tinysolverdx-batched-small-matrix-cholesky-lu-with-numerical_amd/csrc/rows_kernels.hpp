@@ -65,8 +65,9 @@ struct ColumnDivider<double, R> {
 
 // Factor in registers.  r[s][j] = A(q + L*s, j) on entry, L on exit (entries
 // above the diagonal are garbage).  col: per-wave LDS, 2 * 64 * R elements.
+// nloop / n as for chol_regs (plain_kernels.hpp).
 template <class T, int L, int R>
-__device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int n, int q, int g, T* col) {
+__device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int nloop, int n, int q, int g, T* col) {
   constexpr int N = L * R;
   int info = 0;
   auto pivot = [&](auto kc) {
@@ -88,7 +89,7 @@ __device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int n, int q, int g, 
   pivot(ic<0>{});
   static_for<0, N - 1>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    if (k + 1 < n) {
+    if (k + 1 < nloop) {
       T* cb = col + (k & 1) * (kWave * R) + g * N;
       static_for<k / L, R>([&](auto s) { cb[q + L * s] = r[s][k]; });
       wave_sync();
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(kWave * WPB) void k_potrf_rows(T* __restrict__ A, i
     const int row = q + L * s;
     load_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, live && row < n, vec != 0);
   });
-  const int inf = chol_rows<T, L, R>(r, n, q, g, col_s[wv]);
+  const int inf = chol_rows<T, L, R>(r, n, n, q, g, col_s[wv]);
   static_for<0, R>([&](auto s) {
     const int row = q + L * s;
     if (live && row < n) store_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, vec != 0, true, row);

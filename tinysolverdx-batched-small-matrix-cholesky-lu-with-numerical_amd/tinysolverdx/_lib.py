@@ -39,7 +39,15 @@ for _t in ("f32", "f64"):
     })
 SIGNATURES["tsdx_hip_posv_ir_workspace_size"] = [_i64]
 SIGNATURES["tsdx_hip_posv_ir_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, C.c_size_t, _vp]
-_RES = {"tsdx_hip_last_error": C.c_char_p, "tsdx_hip_posv_ir_workspace_size": C.c_size_t}
+for _t in ("f32", "f64"):
+    SIGNATURES[f"tsdx_hip_ragged_plan_create_{_t}"] = [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i, C.POINTER(_vp), _vp]
+SIGNATURES["tsdx_hip_ragged_run"] = [_vp, _vp]
+SIGNATURES["tsdx_hip_ragged_plan_report"] = [_vp]
+SIGNATURES["tsdx_hip_ragged_plan_destroy"] = [_vp]
+_RES = {"tsdx_hip_last_error": C.c_char_p, "tsdx_hip_posv_ir_workspace_size": C.c_size_t,
+        "tsdx_hip_ragged_plan_report": C.c_char_p}
+
+KIND = {"cholesky": 0, "lu": 1}
 
 FACTOR = {"auto": 0, "fp32": 1, "bf16": 2, "fp16": 3, "fp64": 4}
 

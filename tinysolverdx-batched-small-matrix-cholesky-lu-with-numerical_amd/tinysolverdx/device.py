@@ -138,3 +138,46 @@ def posv_ir(A, B, X, factor="fp32", max_iter=10, tol=1e-14, info=None, stat=None
               _chk(info, "info", torch.int32), _chk(stat, "stat", torch.int32), _chk(work, "work"), work.numel(),
               _stream(A))
     return X, info, stat
+
+
+class RaggedPlan:
+    """Matrices of different sizes and kinds factored by one graph launch.
+
+    A is a flat float32 / float64 device tensor; matrix m (row-major n x n) is
+    A[offsets[m] : offsets[m] + n*n] with n = sizes[m]; kinds[m] is 0 (Cholesky)
+    or 1 (LU).  offsets / piv_offsets are int64, sizes / kinds / piv / info
+    int32 device tensors; piv and info may be None.  Each matrix gets the
+    result of the per-size potrf / getrf of its kind, bit for bit.  The plan
+    is bound to these tensors (keep them alive); run() factors A in place on
+    torch's current stream.  tune=True lets the plan time its compiled
+    variants on synthetic matrices of the same sizes and keep the fastest.
+    """
+
+    def __init__(self, A, offsets, sizes, kinds, piv=None, piv_offsets=None, info=None, tune=True):
+        import ctypes as C
+        import json
+        import torch
+        if A.dim() != 1:
+            raise ValueError("A must be a flat tensor")
+        t = _dt(A)
+        self._keep = (A, offsets, sizes, kinds, piv, piv_offsets, info)
+        self._plan = C.c_void_p()
+        _lib.call(f"tsdx_hip_ragged_plan_create_{t}", _chk(A, "A"), _chk(offsets, "offsets", torch.int64),
+                  _chk(sizes, "sizes", torch.int32), _chk(kinds, "kinds", torch.int32), sizes.numel(),
+                  _chk(piv, "piv", torch.int32), _chk(piv_offsets, "piv_offsets", torch.int64),
+                  _chk(info, "info", torch.int32), 1 if tune else 0, C.byref(self._plan), _stream(A))
+        self.report = json.loads(_lib.lib().tsdx_hip_ragged_plan_report(self._plan).decode())
+
+    def run(self):
+        _lib.call("tsdx_hip_ragged_run", self._plan, _stream(self._keep[0]))
+
+    def close(self):
+        if self._plan:
+            _lib.lib().tsdx_hip_ragged_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
