@@ -14,7 +14,12 @@ kernel (HIP events on the launch stream) and the reference's own CPU posv
 (compiled from its sources, oracle/_ref) timed on this host's cores.  Progress
 goes to stderr.
 
-    python bench.py [--steps K] [--warmup W] [--nrhs R] [--batch N] [--extra]
+    python bench.py [--steps K] [--warmup W] [--nrhs R] [--batch N] [--factor F]
+
+The other workloads of the path (one JSON line each, same fields):
+    python bench.py --workload potrf                       fp32 n=32 Cholesky, 65536 matrices
+    python bench.py --workload getrf --n 64 --dtype f64    LU + gesv(4 RHS), ~1 GiB of A
+    python bench.py --workload ragged --dtype f32          1,000,000 mixed sizes and kinds
 """
 import argparse
 import json
@@ -142,12 +147,24 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (reference kernel for scaling rocprof byte counters)")
+    ap.add_argument("--workload", default="posv", choices=["posv", "potrf", "getrf", "ragged"],
+                    help="posv: the headline (default); potrf: fp32 n=32 Cholesky of 65536 matrices; getrf: LU "
+                         "+ gesv with 4 right-hand sides on ~1 GiB of A (--n, --dtype); ragged: 1,000,000 "
+                         "matrices of Zipf sizes 2..64, half Cholesky half LU, one graph launch (--dtype)")
+    ap.add_argument("--n", type=int, default=32, help="matrix size of the getrf workload")
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"], help="type of the getrf / ragged workloads")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the batched solvers shard by "
               "splitting the batch, with no exchange between GPUs", file=sys.stderr)
         return 2
+    run = {"posv": run_posv, "potrf": run_potrf, "getrf": run_getrf, "ragged": run_ragged}[a.workload]
+    out = run(a)
+    print(json.dumps(out), flush=True)
+    return 0
 
+
+def run_posv(a):
     import numpy as np
     import torch
     import tinysolverdx  # noqa: F401  (binds to torch's HIP runtime)
@@ -236,8 +253,265 @@ def main(argv=None):
         log(f"[bench] CPU baseline {out['cpu_baseline']['value']:.3g} solves/s on {out['cpu_baseline']['cores']} cores")
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
+
+
+
+# --------------------------------------------------------------------------
+# The other workloads.  The factorizations work in place, so every step first
+# restores A from a pristine copy and pushes A out of the 256 MB Infinity Cache
+# (an untimed read-modify-write of 1 GiB); the step's time is the HIP-event
+# time of the factorization alone, summed over the K steps.
+
+def _inplace_steps(fn, restore, steps, warmup):
+    import torch
+    flush = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    for _ in range(warmup):
+        restore()
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        restore()
+        flush.add_(1.0)
+        e0.record(s)
+        fn()
+        e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, sum(e0.elapsed_time(e1) for e0, e1 in ev) * 1e-3
+
+
+def _line(metric, value, unit, a, ms, dtype, config, roofline, cpu):
+    return {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config, "roofline": roofline,
+            "cpu_baseline": cpu}
+
+
+def _roof(bytes_total, sec, kernel, per_unit):
+    ach = bytes_total / sec / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel, "bytes_per_matrix": per_unit}
+
+
+def _cpu_timed(fn, units, unit, sample):
+    """Best of up to 5 runs (<= ~20 s) of a reference CPU call on the host cores."""
+    from oracle import available
+    best, total, reps = float("inf"), 0.0, 0
+    while reps < 5 and total < 20.0:
+        t0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - t0
+        best, total, reps = min(best, dt), total + dt, reps + 1
+    return {"value": units / best, "unit": unit, "cores": len(os.sched_getaffinity(0)),
+            "kind": "reference" if available("reference") else "port", "sample": f"{sample}, best of {reps} reps"}
+
+
+def _cpu_oracle():
+    from oracle import Oracle, available
+    orc = Oracle("reference" if available("reference") else "port")
+    orc.set_num_threads(len(os.sched_getaffinity(0)))
+    return orc
+
+
+def make_spd_iid(n, batch, lmin, seed, chunk=16384):
+    """A = Q diag(lambda) Q^T symmetrised, lambda i.i.d. log-uniform on [lmin, 1]."""
+    import math
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A = torch.empty((batch, n, n), dtype=torch.float64, device="cuda")
+    for s in range(0, batch, chunk):
+        e = min(batch, s + chunk)
+        q = haar_q(torch.randn((e - s, n, n), generator=g, device="cuda", dtype=torch.float64))
+        lam = 10.0 ** (math.log10(lmin) * torch.rand((e - s, n), generator=g, device="cuda", dtype=torch.float64))
+        a = torch.matmul(q * lam[:, None, :], q.transpose(1, 2).contiguous())
+        A[s:e] = 0.5 * (a + a.transpose(1, 2))
+    return A
+
+
+def run_potrf(a):
+    """Plain fp32 Cholesky, n = 32, 65536 matrices (lambda log-uniform on [1e-3, 1])."""
+    import torch
+    import tinysolverdx  # noqa: F401
+    from tinysolverdx import device
+    n, batch = 32, 65536
+    log(f"[bench] potrf: generating {batch} SPD n={n} on {torch.cuda.get_device_name(0)}")
+    A0 = make_spd_iid(n, batch, 1e-3, seed=20261203).float()
+    A = A0.clone()
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    wall, ev = _inplace_steps(lambda: device.potrf(A, info), lambda: A.copy_(A0), a.steps, a.warmup)
+    bad = int((info != 0).sum().item())
+    L = A[:256].double()
+    res = (torch.linalg.matrix_norm(L @ L.transpose(1, 2) - A0[:256].double()) /
+           torch.linalg.matrix_norm(A0[:256].double())).max().item()
+    ms = ev / a.steps * 1e3
+    per = 2 * 4 * n * n + 4
+    log(f"[bench] potrf fp32 n={n}: {ms:.3f} ms/launch, info>0: {bad}, max ||A-LL^T||/||A|| (256 sampled) {res:.2e}")
+    cpu = None
+    if not a.no_cpu:
+        orc = _cpu_oracle()
+        Ah = A0.cpu().numpy()
+        cpu = _cpu_timed(lambda: orc.cholesky(Ah), batch, "matrices/s", f"full workload ({batch} matrices)")
+    return _line("potrf matrices/s fp32 n=32 (65536 SPD); HBM GB/s", batch / (ms * 1e-3), "matrices/s", a, ms,
+                 "f32", {"workload": f"potrf fp32 n={n}, batch={batch}, Q diag(lambda) Q^T, lambda log-uniform "
+                                     "[1e-3,1]", "info_gt0": bad, "max_rel_residual_sampled": res,
+                         "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)},
+                 _roof(per * batch, ms * 1e-3, "k_potrf_rows<float,8,4,1>", per), cpu)
+
+
+def run_getrf(a):
+    """LU with partial pivoting + gesv (4 right-hand sides), ~1 GiB of A, 1% zero columns."""
+    import torch
+    import tinysolverdx  # noqa: F401
+    from tinysolverdx import device
+    n = a.n
+    dt = torch.float32 if a.dtype == "f32" else torch.float64
+    es = 4 if a.dtype == "f32" else 8
+    batch = (1 << 30) // (es * n * n)
+    g = torch.Generator(device="cuda").manual_seed(20261204)
+    log(f"[bench] getrf: {batch} N(0,1) {a.dtype} n={n}")
+    A0 = torch.randn((batch, n, n), generator=g, device="cuda", dtype=dt)
+    zc = torch.nonzero(torch.rand(batch, generator=g, device="cuda") < 0.01)[:, 0]
+    cols = torch.randint(0, n, (zc.numel(),), generator=g, device="cuda")
+    A0[zc, :, cols] = 0.0
+    A = A0.clone()
+    piv = torch.zeros((batch, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    wall, ev = _inplace_steps(lambda: device.getrf(A, piv, info), lambda: A.copy_(A0), a.steps, a.warmup)
+    ms = ev / a.steps * 1e3
+    bad = int((info != 0).sum().item())
+    B = torch.randn((batch, n, 4), generator=g, device="cuda", dtype=dt)
+    X = torch.empty_like(B)
+    wall_s, ev_s = time_steps(lambda: device.gesv_solve(A0, B, X, piv, info), a.steps, a.warmup)
+    ms_s = ev_s / a.steps * 1e3
+    per = 2 * es * n * n + 4 * n + 4
+    per_s = es * n * n + 2 * es * n * 4 + 4 * n + 4
+    log(f"[bench] getrf {a.dtype} n={n}: {ms:.3f} ms, gesv(4) {ms_s:.3f} ms, info>0: {bad} "
+        f"(zeroed columns {zc.numel()})")
+    cpu = None
+    if not a.no_cpu:
+        orc = _cpu_oracle()
+        take = min(batch, max(1024, int(4e8 // (n ** 3))))  # a bounded sample
+        Ah = A0[:take].cpu().numpy()
+        cpu = _cpu_timed(lambda: orc.lu(Ah), take, "matrices/s", f"first {take} of the {batch} matrices")
+    cfg = {"workload": f"getrf {a.dtype} n={n}, batch={batch} (~1 GiB of A), N(0,1), 1% zeroed columns",
+           "info_gt0": bad, "zeroed_columns": int(zc.numel()),
+           "gesv4_matrices_per_s": round(batch / (ms_s * 1e-3), 1),
+           "gesv4_GBps": round(per_s * batch / (ms_s * 1e-3) / 1e9, 1),
+           "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)}
+    return _line(f"getrf matrices/s {a.dtype} n={n} (~1 GiB of A); HBM GB/s", batch / (ms * 1e-3), "matrices/s",
+                 a, ms, a.dtype, cfg, _roof(per * batch, ms * 1e-3, f"k_getrf<{'float' if es == 4 else 'double'},"
+                                                                    f"{max(4, 1 << (n - 1).bit_length())}>", per), cpu)
+
+
+def ragged_inputs(count, dtype, seed):
+    """Zipf sizes on 2..64 (P(n) ~ 1/n), half SPD (kappa log-uniform [1, 1e4]),
+    half N(0,1), packed with every matrix starting on a 16-byte boundary."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    ns = np.arange(2, 65)
+    p = 1.0 / ns
+    sizes = rng.choice(ns, size=count, p=p / p.sum()).astype(np.int32)
+    kinds = (rng.random(count) < 0.5).astype(np.int32)
+    al = 4
+    span = ((sizes.astype(np.int64) ** 2 + al - 1) // al) * al
+    offs = np.zeros(count, np.int64)
+    offs[1:] = np.cumsum(span)[:-1]
+    poffs = np.zeros(count, np.int64)
+    poffs[1:] = np.cumsum(sizes.astype(np.int64))[:-1]
+    tdt = torch.float32 if dtype == "f32" else torch.float64
+    A = torch.zeros(int(span.sum()), dtype=tdt, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    d_off = torch.from_numpy(offs).cuda()
+    for kind in (0, 1):
+        for n in range(2, 65):
+            ids = np.nonzero((sizes == n) & (kinds == kind))[0]
+            if ids.size == 0:
+                continue
+            if kind == 0:
+                M, _ = make_spd(n, ids.size, 1.0 + 1e-9, 1e4, seed=seed + n)
+            else:
+                M = torch.randn((ids.size, n, n), generator=g, device="cuda", dtype=torch.float64)
+            idx = d_off[torch.from_numpy(ids).cuda()][:, None] + torch.arange(n * n, device="cuda")[None, :]
+            A[idx.reshape(-1)] = M.reshape(-1).to(tdt)
+    return A, sizes, kinds, offs, poffs
+
+
+def run_ragged(a):
+    """1,000,000 matrices, Zipf sizes 2..64, half Cholesky half LU, one graph launch."""
+    import numpy as np
+    import torch
+    import tinysolverdx  # noqa: F401
+    from tinysolverdx import device
+    count = a.batch if a.batch != 131072 else 1_000_000
+    t0 = time.perf_counter()
+    A0, sizes, kinds, offs, poffs = ragged_inputs(count, a.dtype, 20261206)
+    log(f"[bench] ragged {a.dtype}: {count} matrices generated in {time.perf_counter() - t0:.1f}s")
+    A = A0.clone()
+    d_off, d_poff = torch.from_numpy(offs).cuda(), torch.from_numpy(poffs).cuda()
+    d_size, d_kind = torch.from_numpy(sizes).cuda(), torch.from_numpy(kinds).cuda()
+    piv = torch.zeros(int(sizes.sum()), dtype=torch.int32, device="cuda")
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    t0 = time.perf_counter()
+    plan = device.RaggedPlan(A, d_off, d_size, d_kind, piv, d_poff, info, tune=True)
+    log(f"[bench] plan + autotune {time.perf_counter() - t0:.1f}s: {plan.report}")
+    wall, ev = _inplace_steps(plan.run, lambda: A.copy_(A0), a.steps, a.warmup)
+    ms = ev / a.steps * 1e3
+    es = 4 if a.dtype == "f32" else 8
+    n64 = sizes.astype(np.int64)
+    total = int((2 * es * n64 * n64 + 4 * n64 * kinds + 4).sum())
+    bad = int((info != 0).sum().item())
+    # the same matrices as per-size launches (one launch per kind and n), for comparison
+    per_ms = 0.0
+    s = torch.cuda.current_stream()
+    for kind in (0, 1):
+        for n in range(2, 65):
+            ids = np.nonzero((sizes == n) & (kinds == kind))[0]
+            if ids.size == 0:
+                continue
+            idx = d_off[torch.from_numpy(ids).cuda()][:, None] + torch.arange(n * n, device="cuda")[None, :]
+            D = A0[idx.reshape(-1)].reshape(ids.size, n, n)
+            I = torch.zeros(ids.size, dtype=torch.int32, device="cuda")
+            P = torch.zeros((ids.size, n), dtype=torch.int32, device="cuda")
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            if kind == 0:
+                device.potrf(D, I)
+            else:
+                device.getrf(D, P, I)
+            e1.record(s)
+            torch.cuda.synchronize()
+            per_ms += e0.elapsed_time(e1)
+    log(f"[bench] ragged {a.dtype}: {ms:.3f} ms per graph launch, per-size launches {per_ms:.3f} ms, info>0: {bad}")
+    cpu = None
+    if not a.no_cpu:
+        orc = _cpu_oracle()
+        take = min(count, 100_000)
+        Ah = A0.cpu().numpy()
+        groups = []
+        for kind in (0, 1):
+            for n in range(2, 65):
+                ids = np.nonzero((sizes[:take] == n) & (kinds[:take] == kind))[0]
+                if ids.size:
+                    groups.append((kind, np.stack([Ah[offs[m]:offs[m] + n * n].reshape(n, n) for m in ids])))
+
+        def cpu_run():
+            for kind, M in groups:
+                (orc.cholesky if kind == 0 else orc.lu)(M)
+        cpu = _cpu_timed(cpu_run, take, "matrices/s", f"first {take} of the {count} matrices, one call per kind and n")
+    plan.close()
+    cfg = {"workload": f"ragged {a.dtype}: {count} matrices, n Zipf on 2..64 (P(n) ~ 1/n), half SPD Cholesky "
+                       "(kappa log-uniform [1,1e4]) half N(0,1) LU, one graph launch",
+           "plan": {k: v for k, v in plan.report.items()}, "info_gt0": bad,
+           "per_size_launches_ms": round(per_ms, 4), "graph_over_per_size": round(per_ms / ms, 3),
+           "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)}
+    return _line(f"ragged factor matrices/s {a.dtype} (1M, n 2..64, half Cholesky half LU); HBM GB/s",
+                 count / (ms * 1e-3), "matrices/s", a, ms, a.dtype, cfg,
+                 _roof(total, ms * 1e-3, "k_ragged<T,class,variant> x3 (one graph)", round(total / count, 1)), cpu)
 
 
 if __name__ == "__main__":
