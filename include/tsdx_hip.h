@@ -123,6 +123,41 @@ int tsdx_hip_posv_ir_f64(const double *A, int n, int64_t batch, const double *B,
                          int max_iter, double tol, int *info, int *stat, void *work, size_t work_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Learned precision router for posv (no reference counterpart).  A small
+ * network picks, per matrix, the cheapest tier expected to reach the
+ * tolerance: 0 = bf16 factor + float64 refinement, 1 = fp32 factor + float64
+ * refinement, 2 = float64 direct.  Every answer is still judged by the float64
+ * residual; refinement that does not converge falls back to float64.
+ *   features  (count, 12) float: one pass over the lower triangle of A.
+ *   params    TSDX_HIP_ROUTER_PARAMS floats: feature means (12) and scales
+ *             (12), then W1 (64x12), b1, W2 (64x64), b2, W3 (3x64), b3,
+ *             row-major [out][in]; ReLU hidden layers.
+ * Training: tsdx_hip_router_grad gives the gradient of the mean softmax
+ * cross-entropy (and the loss) over a batch of (features, label); the first
+ * 24 gradient entries are zero.  tsdx_hip_router_adam applies one Adam step
+ * (beta1 0.9, beta2 0.999, eps 1e-8; step counts from 1) to W1..b3.
+ * ------------------------------------------------------------------------ */
+#define TSDX_HIP_ROUTER_FEATURES 12
+#define TSDX_HIP_ROUTER_TIERS 3
+#define TSDX_HIP_ROUTER_PARAMS 5211
+
+int tsdx_hip_router_features_f64(const double *A, int n, int64_t batch, float *feat, void *stream);
+int tsdx_hip_router_forward(const float *params, const float *feat, int64_t count, float *logits, void *stream);
+size_t tsdx_hip_router_grad_workspace_size(int64_t count);
+int tsdx_hip_router_grad(const float *params, const float *feat, const int *label, int64_t count, float *grad,
+                         float *loss, void *work, size_t work_bytes, void *stream);
+int tsdx_hip_router_adam(float *params, const float *grad, float *m, float *v, int step, float lr, void *stream);
+
+/* Routed posv (float64 A, B, X): features, network, the refinement tiers over
+ * their matrices, then the float64 solve of tier-2 matrices and of every
+ * fallback.  stat as for tsdx_hip_posv_ir_f64; tier (batch) or NULL receives
+ * the network's choice.  Only the lower triangle of A is read. */
+size_t tsdx_hip_posv_routed_workspace_size(int64_t batch);
+int tsdx_hip_posv_routed_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs,
+                             const float *params, int max_iter, double tol, int *info, int *stat, int *tier,
+                             void *work, size_t work_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Ragged batches: matrices of different sizes (1..64) and kinds in one call.
  * No reference counterpart (the reference drivers take one n per call,
  * batched.hpp:121-244); each matrix gets exactly the result of the per-size

@@ -1,0 +1,180 @@
+"""Learned precision router: HIP features / forward / gradient / Adam against
+the float64 CPU oracle (oracle/mlp_oracle.py, itself pinned by finite
+differences), training, and the routed posv.
+
+The gain test trains on separately seeded matrices and compares the routed
+posv with the un-routed fp32 refinement on fresh ones: both must meet the
+1e-14 backward-error bar everywhere; the routed one must waste far fewer
+refinement attempts on matrices that end in the float64 fallback.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _spd(rng, batch, n, lo, hi):
+    from tinysolverdx import matrices
+    kappa = 10.0 ** rng.uniform(lo, hi, batch)
+    return matrices.spd_kappas(rng, kappa, n), kappa
+
+
+def test_features_match_oracle():
+    import torch
+    import mlp_oracle as M
+    from tinysolverdx import router
+    rng = np.random.default_rng(1)
+    for n in (3, 16, 33, 64):
+        A, _ = _spd(rng, 20, n, 0, 10)
+        A[3] = -A[3]                      # negative diagonal
+        A[5] = np.diag(rng.uniform(1, 2, n))
+        iu = np.triu_indices(n, 1)
+        A[7][iu] = 1e20                   # upper triangle must not matter
+        got = router.features(torch.from_numpy(A).cuda()).cpu().numpy()
+        want = M.features(A)
+        assert np.allclose(got, want, rtol=1e-4, atol=2e-4), (n, np.abs(got - want).max(axis=0))
+
+
+def test_forward_and_gradient_match_oracle():
+    import torch
+    import mlp_oracle as M
+    from tinysolverdx import router
+    rng = np.random.default_rng(2)
+    p = router.init_params(3)
+    p[0:12] = rng.standard_normal(12)
+    p[12:24] = rng.uniform(0.5, 2, 12)
+    p[router.OFF_B1:router.OFF_W2] = rng.standard_normal(64) * 0.1
+    r = router.Router(p)
+    for count in (1, 63, 64, 1000):  # partial and several chunks
+        feat = rng.standard_normal((count, 12)).astype(np.float32)
+        label = rng.integers(0, 3, count).astype(np.int32)
+        f, l = torch.from_numpy(feat).cuda(), torch.from_numpy(label).cuda()
+        z = r.forward(f).cpu().numpy()
+        zo = M.forward(p, feat)[3]
+        assert np.allclose(z, zo, rtol=1e-4, atol=1e-4), np.abs(z - zo).max()
+        g, loss = r.grad(f, l)
+        go = M.grad(p, feat, label)
+        g = g.cpu().numpy()
+        assert np.all(g[:24] == 0)
+        scale = np.abs(go).max()
+        assert np.abs(g - go).max() <= 1e-4 * scale + 1e-6, (count, np.abs(g - go).max(), scale)
+        assert abs(loss.item() - M.loss(p, feat, label)) < 1e-4
+
+
+def test_adam_step_matches_oracle():
+    import torch
+    import mlp_oracle as M
+    from tinysolverdx import _lib, router
+    rng = np.random.default_rng(4)
+    p = router.init_params(5)
+    g = rng.standard_normal(router.PARAMS).astype(np.float32)
+    g[:24] = 0
+    m = rng.standard_normal(router.PARAMS).astype(np.float32) * 0.1
+    v = rng.uniform(0, 0.1, router.PARAMS).astype(np.float32)
+    dp, dg, dm, dv = (torch.from_numpy(x.copy()).cuda() for x in (p, g, m, v))
+    _lib.call("tsdx_hip_router_adam", dp.data_ptr(), dg.data_ptr(), dm.data_ptr(), dv.data_ptr(), 7, 1e-3,
+              torch.cuda.current_stream().cuda_stream)
+    # the kernel's constants are float32 (1 - beta2 differs from 1e-3 by 1.3e-5 relative)
+    b1, b2 = float(np.float32(0.9)), float(np.float32(0.999))
+    po, mo, vo = M.adam(p, g.astype(np.float64), m, v, 7, float(np.float32(1e-3)), b1=b1, b2=b2)
+    assert np.allclose(dp.cpu().numpy(), po, rtol=1e-5, atol=1e-6)
+    assert np.allclose(dm.cpu().numpy(), mo, rtol=1e-5, atol=1e-7)
+    assert np.allclose(dv.cpu().numpy(), vo, rtol=1e-5, atol=1e-9)
+    assert np.array_equal(dp.cpu().numpy()[:24], p[:24])
+
+
+def test_training_learns_a_separable_rule():
+    """Label = which of three feature bands feature 1 falls in; the loss must
+    fall and the accuracy be high after a short fit."""
+    import torch
+    from tinysolverdx import router
+    rng = np.random.default_rng(6)
+    feat = rng.standard_normal((8192, 12)).astype(np.float32)
+    label = np.digitize(feat[:, 1], [-0.5, 0.5]).astype(np.int32)
+    f, l = torch.from_numpy(feat).cuda(), torch.from_numpy(label).cuda()
+    r = router.Router(seed=1)
+    hist = r.fit(f, l, epochs=30, batch=1024, lr=3e-3)
+    assert hist[-1] < 0.5 * hist[0], hist
+    acc = (r.predict(f).cpu().numpy() == label).mean()
+    assert acc > 0.95, acc
+
+
+def _bwd(tsdx, A, X, B):
+    return np.stack([tsdx.solve_backward_error(A, X[..., r], B[..., r]) for r in range(B.shape[-1])]).max(axis=0)
+
+
+def test_routed_posv_every_answer_meets_tolerance(tsdx):
+    import torch
+    from tinysolverdx import router
+    rng = np.random.default_rng(7)
+    A, _ = _spd(rng, 96, 32, 0, 12)
+    A[10] = -np.eye(32)  # not SPD: info like the plain posv
+    B = rng.standard_normal((96, 32, 2))
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    for tiers in (0, 1, 2, None):
+        p = router.init_params(0)
+        if tiers is not None:  # a network that always answers `tiers`
+            p[router.OFF_W3:router.OFF_B3] = 0
+            p[router.OFF_B3:router.OFF_B3 + 3] = 0
+            p[router.OFF_B3 + tiers] = 1.0
+        r = router.Router(p)
+        X = torch.zeros_like(dB)
+        info = torch.full((96,), -9, dtype=torch.int32, device="cuda")
+        stat = torch.full((96,), -9, dtype=torch.int32, device="cuda")
+        tier = torch.full((96,), -9, dtype=torch.int32, device="cuda")
+        r.posv(dA, dB, X, info=info, stat=stat, tier=tier)
+        torch.cuda.synchronize()
+        info, stat, tier = info.cpu().numpy(), stat.cpu().numpy(), tier.cpu().numpy()
+        assert set(np.unique(tier)) <= {0, 1, 2}
+        if tiers is not None:
+            assert (tier == tiers).all()
+        if tiers == 2:
+            assert (stat == -1).all()
+        assert info[10] == 1 and (np.delete(info, 10) == 0).all()
+        ok = info == 0
+        Xh = X.cpu().numpy()
+        assert _bwd(tsdx, A[ok], Xh[ok], B[ok]).max() <= 1e-14, tiers
+
+
+def test_router_gain_over_unrouted(tsdx):
+    """Trained on one seed, evaluated on another (kappa log-uniform [1, 1e12],
+    n in {16, 32}): routed and un-routed both meet 1e-14 everywhere, and the
+    routed solve wastes far fewer refinement attempts on matrices that need
+    the float64 solve anyway."""
+    import torch
+    from tinysolverdx import device, router
+    rng = np.random.default_rng(8)
+    feats, labels = [], []
+    for n in (16, 32):
+        A, _ = _spd(rng, 3000, n, 0, 12)
+        dA = torch.from_numpy(A).cuda()
+        dB = torch.from_numpy(rng.standard_normal((3000, n, 1))).cuda()
+        labels.append(router.make_labels(dA, dB))
+        feats.append(router.features(dA))
+    r = router.Router(seed=2)
+    hist = r.fit(torch.cat(feats), torch.cat(labels), epochs=60, batch=1000, lr=3e-3)
+    assert hist[-1] < hist[0]
+    rng = np.random.default_rng(9)
+    wasted_routed = wasted_plain = needed64 = overuse = solvable = 0
+    for n in (16, 32):
+        A, _ = _spd(rng, 1500, n, 0, 12)
+        B = rng.standard_normal((1500, n, 1))
+        dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+        X1, X2 = torch.zeros_like(dB), torch.zeros_like(dB)
+        s1 = torch.zeros(1500, dtype=torch.int32, device="cuda")
+        s2 = torch.zeros_like(s1)
+        tier = torch.zeros_like(s1)
+        device.posv_ir(dA, dB, X1, factor="fp32", stat=s1)
+        r.posv(dA, dB, X2, stat=s2, tier=tier)
+        torch.cuda.synchronize()
+        s1, s2, tier = s1.cpu().numpy(), s2.cpu().numpy(), tier.cpu().numpy()
+        assert _bwd(tsdx, A, X1.cpu().numpy(), B).max() <= 1e-14
+        assert _bwd(tsdx, A, X2.cpu().numpy(), B).max() <= 1e-14
+        needed64 += int((s1 == -1).sum())
+        wasted_plain += int((s1 == -1).sum())               # every fp32 failure was a wasted attempt
+        wasted_routed += int(((s2 == -1) & (tier != 2)).sum())  # attempts routed to a tier that then failed
+        solvable += int((s1 >= 0).sum())
+        overuse += int(((s1 >= 0) & (tier == 2)).sum())   # sent to float64 although fp32 refinement converges
+    assert needed64 > 300
+    assert wasted_routed < 0.5 * wasted_plain, (wasted_routed, wasted_plain)
+    assert overuse < 0.3 * solvable, (overuse, solvable)

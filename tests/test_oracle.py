@@ -175,3 +175,37 @@ def test_error_codes(oracle):
         oracle.cholesky(np.zeros((1, 65, 65)))
     with pytest.raises(RuntimeError):
         oracle.cholesky(np.eye(3)[None], "cuda")
+
+
+def test_router_oracle_gradient_matches_finite_differences():
+    """The CPU MLP that pins the router kernels: its analytic gradient against
+    central differences of its loss, at entries of every layer."""
+    import mlp_oracle as M
+    rng = np.random.default_rng(0)
+    p = rng.standard_normal(M.PARAMS) * 0.3
+    p[12:24] = rng.uniform(0.5, 2.0, 12)
+    feat = rng.standard_normal((40, 12))
+    label = rng.integers(0, 3, 40)
+    g = M.grad(p, feat, label)
+    assert np.all(g[:24] == 0)
+    idx = np.concatenate([rng.integers(lo, hi, 6) for lo, hi in
+                          ((M.OFF_W1, M.OFF_B1), (M.OFF_B1, M.OFF_W2), (M.OFF_W2, M.OFF_B2), (M.OFF_B2, M.OFF_W3),
+                           (M.OFF_W3, M.OFF_B3), (M.OFF_B3, M.PARAMS))])
+    fd = M.finite_difference(p, feat, label, idx)
+    assert np.allclose(g[idx], fd, rtol=1e-5, atol=1e-8), np.abs(g[idx] - fd).max()
+    # a wrong sign or a dropped ReLU mask would fail here: the loss decreases along -g
+    assert M.loss(p - 1e-3 * g, feat, label) < M.loss(p, feat, label)
+
+
+def test_router_oracle_features_closed_forms():
+    """Features of diagonal and 2x2 matrices, worked by hand."""
+    import mlp_oracle as M
+    f = M.features(np.diag([4.0, 1.0, 0.25, 1.0])[None])[0]
+    assert f[0] == 4 / 64 and np.isclose(f[1], np.log10(0.25)) and np.isclose(f[2], np.log10(4.0))
+    assert np.isclose(f[3], np.log10(np.sqrt(16 + 1 + 1 / 16 + 1))) and np.isclose(f[7], np.log10(16.0))
+    assert f[6] == -30.0 and f[10] == 0.0 and f[11] == 1.0
+    # only the lower triangle counts: the upper 99 is ignored
+    a = np.array([[2.0, 99.0], [-1.0, 3.0]])
+    f = M.features(a[None])[0]
+    assert np.isclose(f[4], np.log10(4.0)) and np.isclose(f[5], (2.0 - 1.0) / np.sqrt(4 + 9 + 2)) and f[11] == 1.0
+    assert M.features(np.array([[[-1.0, 0.0], [0.0, 1.0]]]))[0][10] == 1.0

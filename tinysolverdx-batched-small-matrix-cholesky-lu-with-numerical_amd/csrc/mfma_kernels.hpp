@@ -218,7 +218,8 @@ __global__ __launch_bounds__(kWave) void k_posv_mx(IrArgs a) {
   __shared__ __align__(16) double xs[NP];
   __shared__ __align__(16) float fs[16];
   const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
-  const int64_t m = blockIdx.x;
+  if (a.list && (int)blockIdx.x >= *a.lcount) return;  // uniform: the whole workgroup leaves
+  const int64_t m = a.list ? a.list[blockIdx.x] : blockIdx.x;
   const int n = a.n;
   const double* Am = a.A + m * (int64_t)n * n;
 

@@ -19,9 +19,15 @@ struct IrArgs {
   int* stat;    // (batch) or null: corrections used, or -1 when sent to the float64 solve
   int* flist;   // (batch) matrices handed to the float64 solve
   int* fcount;  // (1) their number
+  // Optional subset for the refinement kernel: when list is set, workgroup w
+  // solves matrix list[w] for w < *lcount (the router's tier lists).
+  const int* list = nullptr;
+  const int* lcount = nullptr;
 };
 
 // low: 1 float, 2 bfloat16, 3 half (refinement), 4 float64 direct for all.
 hipError_t launch_posv_ir(const IrArgs& a, int low, hipStream_t s);
+// float64 posv of the matrices on a.flist[0 .. *a.fcount).
+hipError_t launch_posv_gather(const IrArgs& a, hipStream_t s);
 
 }  // namespace tsdx_hip

@@ -58,18 +58,4 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_posv_gather(I
   }
 }
 
-__global__ void k_zero_int(int* p) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
-}
-
-// list[i] = i for i < batch, count = batch (every matrix takes the float64 solve)
-__global__ void k_iota(int* list, int* count, int* stat, int64_t batch) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < batch) {
-    list[i] = (int)i;
-    if (stat) stat[i] = -1;
-  }
-  if (i == 0) *count = (int)batch;
-}
-
 }  // namespace tsdx_hip

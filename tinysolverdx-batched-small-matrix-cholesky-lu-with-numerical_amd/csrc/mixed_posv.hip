@@ -5,6 +5,20 @@
 
 namespace tsdx_hip {
 
+__global__ void k_zero_int(int* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+// list[i] = i for i < batch, count = batch (every matrix takes the float64 solve)
+__global__ void k_iota(int* list, int* count, int* stat, int64_t batch) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < batch) {
+    list[i] = (int)i;
+    if (stat) stat[i] = -1;
+  }
+  if (i == 0) *count = (int)batch;
+}
+
 // Matrix-core refinement kernel: one wave (one workgroup) per matrix.
 template <int NBLK>
 static void launch_mx(const IrArgs& a, int low, hipStream_t s) {
@@ -17,6 +31,26 @@ static void launch_mx(const IrArgs& a, int low, hipStream_t s) {
     hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F32>), grid, block, 0, s, a);
 }
 
+// float64 posv of the matrices on the fallback list (grid-stride over it)
+template <int NB>
+static hipError_t gather(const IrArgs& a, hipStream_t s) {
+  const dim3 g = grid_for<NB>(a.batch);
+  const dim3 gg(g.x < 16384u ? g.x : 16384u);
+  if (a.nrhs <= 1)
+    hipLaunchKernelGGL((k_posv_gather<NB, 1>), gg, block_for<NB>(), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_posv_gather<NB, 4>), gg, block_for<NB>(), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_posv_gather(const IrArgs& a, hipStream_t s) {
+  if (a.n <= 4) return gather<4>(a, s);
+  if (a.n <= 8) return gather<8>(a, s);
+  if (a.n <= 16) return gather<16>(a, s);
+  if (a.n <= 32) return gather<32>(a, s);
+  return gather<64>(a, s);
+}
+
 template <int NB>
 static hipError_t launch_ir(const IrArgs& a, int low, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, a.fcount);
@@ -25,13 +59,7 @@ static hipError_t launch_ir(const IrArgs& a, int low, hipStream_t s) {
   } else {  // float64 direct for every matrix: the list is 0..batch-1
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((a.batch + 255) / 256)), dim3(256), 0, s, a.flist, a.fcount, a.stat, a.batch);
   }
-  const dim3 g = grid_for<NB>(a.batch);
-  const dim3 gg(g.x < 16384u ? g.x : 16384u);
-  if (a.nrhs <= 1)
-    hipLaunchKernelGGL((k_posv_gather<NB, 1>), gg, block_for<NB>(), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_posv_gather<NB, 4>), gg, block_for<NB>(), 0, s, a);
-  return hipGetLastError();
+  return gather<NB>(a, s);
 }
 
 hipError_t launch_posv_ir(const IrArgs& a, int low, hipStream_t s) {

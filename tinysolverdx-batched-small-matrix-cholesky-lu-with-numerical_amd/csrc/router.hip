@@ -1,0 +1,76 @@
+// Launchers of the router kernels (router_kernels.hpp) and of the routed posv.
+#include "capi_internal.hpp"
+#include "mfma_kernels.hpp"
+#include "router_api.hpp"
+#include "router_kernels.hpp"
+
+namespace tsdx_hip {
+
+hipError_t launch_router_features(const double* A, int n, int64_t batch, float* feat, hipStream_t s) {
+  const int64_t blocks = (batch + kFeatWPB - 1) / kFeatWPB;
+  hipLaunchKernelGGL(k_router_features, dim3((unsigned)blocks), dim3(kWave * kFeatWPB), 0, s, A, n, batch, feat);
+  return hipGetLastError();
+}
+
+hipError_t launch_router_forward(const float* P, const float* feat, int64_t count, float* logits, hipStream_t s) {
+  const int64_t blocks = (count + kMlpThreads - 1) / kMlpThreads;
+  hipLaunchKernelGGL(k_router_forward, dim3((unsigned)blocks), dim3(kMlpThreads), 0, s, P, feat, count, logits);
+  return hipGetLastError();
+}
+
+size_t router_grad_workspace(int64_t count) {
+  const int64_t nblk = (count + kGradChunk - 1) / kGradChunk;
+  return (size_t)nblk * (kRouterParams + 1) * sizeof(float) + 256;
+}
+
+hipError_t launch_router_grad(const float* P, const float* feat, const int* label, int64_t count, float* grad,
+                              float* loss, void* work, hipStream_t s) {
+  const int64_t nblk = (count + kGradChunk - 1) / kGradChunk;
+  float* partial = static_cast<float*>(work);
+  float* lpart = partial + nblk * kRouterParams;
+  hipLaunchKernelGGL(k_router_grad_partial, dim3((unsigned)nblk), dim3(kGradChunk), 0, s, P, feat, label, count,
+                     partial, lpart);
+  hipLaunchKernelGGL(k_router_reduce, dim3((kRouterParams + 255) / 256), dim3(256), 0, s, partial, lpart, (int)nblk,
+                     1.0f / (float)count, grad, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_router_adam(float* P, const float* grad, float* m, float* v, int step, float lr, hipStream_t s) {
+  hipLaunchKernelGGL(k_router_adam, dim3((kRouterParams - kOffW1 + 255) / 256), dim3(256), 0, s, P, grad, m, v, step,
+                     lr);
+  return hipGetLastError();
+}
+
+__global__ void k_zero_counts(int* p, int k) {
+  if ((int)threadIdx.x < k) p[threadIdx.x] = 0;
+}
+
+template <int NBLK>
+static void routed_mx(IrArgs a, const int* list0, const int* c0, const int* list1, const int* c1, hipStream_t s) {
+  const dim3 grid((unsigned)a.batch), block(kWave);
+  a.list = list0;
+  a.lcount = c0;
+  hipLaunchKernelGGL((k_posv_mx<NBLK, Low::BF16>), grid, block, 0, s, a);
+  a.list = list1;
+  a.lcount = c1;
+  hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F32>), grid, block, 0, s, a);
+}
+
+hipError_t launch_posv_routed(const IrArgs& a, const float* P, float* feat, int* tier, int* list0, int* list1,
+                              int* counts, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, counts, 2);
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, a.fcount, 1);
+  hipError_t e = launch_router_features(a.A, a.n, a.batch, feat, s);
+  if (e != hipSuccess) return e;
+  const int64_t blocks = (a.batch + kMlpThreads - 1) / kMlpThreads;
+  hipLaunchKernelGGL(k_router_route, dim3((unsigned)blocks), dim3(kMlpThreads), 0, s, P, feat, a.batch, tier, list0,
+                     list1, counts, a.flist, a.fcount, a.stat);
+  const int nblk = (a.n + 15) / 16;
+  if (nblk == 1) routed_mx<1>(a, list0, counts, list1, counts + 1, s);
+  else if (nblk == 2) routed_mx<2>(a, list0, counts, list1, counts + 1, s);
+  else if (nblk == 3) routed_mx<3>(a, list0, counts, list1, counts + 1, s);
+  else routed_mx<4>(a, list0, counts, list1, counts + 1, s);
+  return launch_posv_gather(a, s);
+}
+
+}  // namespace tsdx_hip

@@ -1,0 +1,164 @@
+"""Learned precision router for posv, on device-resident torch tensors.
+
+A network of 12 -> 64 -> 64 -> 3 (ReLU, float32 weights) predicts, for each
+SPD matrix, the cheapest posv tier expected to reach the tolerance:
+    0  bf16 factor + float64 refinement
+    1  fp32 factor + float64 refinement
+    2  float64 direct solve
+from 12 features taken in one pass over the lower triangle of A.  Features,
+forward pass, gradient and Adam step are the library's HIP kernels
+(include/tsdx_hip.h, tsdx_hip_router_*); torch only holds the buffers and
+shuffles minibatches.  Labels come from the refinement pipeline itself
+(make_labels): the cheapest tier whose refinement converged on that matrix.
+
+The routed solve (Router.posv) never trusts the network for accuracy: every
+answer is judged by the float64 residual, refinement that stalls falls back
+to the float64 solve, exactly as tsdx_hip_posv_ir_f64.
+"""
+import numpy as np
+
+from . import _lib
+from .device import _chk, _shape, _stream
+
+FEATURES, HIDDEN, TIERS = 12, 64, 3
+PARAMS = 5211
+OFF_W1 = 24
+OFF_B1 = OFF_W1 + HIDDEN * FEATURES
+OFF_W2 = OFF_B1 + HIDDEN
+OFF_B2 = OFF_W2 + HIDDEN * HIDDEN
+OFF_W3 = OFF_B2 + HIDDEN
+OFF_B3 = OFF_W3 + TIERS * HIDDEN
+TIER_NAMES = ("bf16+IR", "fp32+IR", "fp64")
+
+
+def features(A):
+    """(batch, 12) float32 features of float64 A (batch, n, n), on A's device."""
+    import torch
+    batch, n = _shape(A)
+    feat = torch.empty((batch, FEATURES), dtype=torch.float32, device=A.device)
+    _lib.call("tsdx_hip_router_features_f64", _chk(A, "A", torch.float64), n, batch, _chk(feat, "feat"), _stream(A))
+    return feat
+
+
+def make_labels(A, B, max_iter=10, tol=1e-14):
+    """Cheapest converging tier per matrix: 0 if the bf16 refinement converges,
+    else 1 if the fp32 refinement does, else 2 (float64)."""
+    import torch
+    from . import device
+    batch = A.shape[0]
+    X = torch.empty_like(B)
+    work = device.posv_ir_workspace(batch, A.device)
+    st = {}
+    for f in ("bf16", "fp32"):
+        s = torch.empty(batch, dtype=torch.int32, device=A.device)
+        device.posv_ir(A, B, X, factor=f, max_iter=max_iter, tol=tol, stat=s, work=work)
+        st[f] = s
+    label = torch.full((batch,), 2, dtype=torch.int32, device=A.device)
+    label[st["fp32"] >= 0] = 1
+    label[st["bf16"] >= 0] = 0
+    return label
+
+
+def init_params(seed=0):
+    """He-initialised weights, zero biases, identity feature scaling (numpy)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(PARAMS, np.float32)
+    p[12:24] = 1.0
+    p[OFF_W1:OFF_B1] = rng.standard_normal(HIDDEN * FEATURES) * np.sqrt(2.0 / FEATURES)
+    p[OFF_W2:OFF_B2] = rng.standard_normal(HIDDEN * HIDDEN) * np.sqrt(2.0 / HIDDEN)
+    p[OFF_W3:OFF_B3] = rng.standard_normal(TIERS * HIDDEN) * np.sqrt(2.0 / HIDDEN)
+    return p
+
+
+class Router:
+    """Parameters live in a float32 device tensor of PARAMS entries."""
+
+    def __init__(self, params=None, device="cuda", seed=0):
+        import torch
+        p = init_params(seed) if params is None else np.asarray(params, np.float32)
+        if p.shape != (PARAMS,):
+            raise ValueError(f"router parameters must have {PARAMS} entries")
+        self.params = torch.from_numpy(p.copy()).to(device)
+
+    # ---------------------------------------------------------------- inference
+    def forward(self, feat):
+        import torch
+        count = feat.shape[0]
+        logits = torch.empty((count, TIERS), dtype=torch.float32, device=feat.device)
+        if count:
+            _lib.call("tsdx_hip_router_forward", _chk(self.params, "params", torch.float32),
+                      _chk(feat, "feat", torch.float32), count, _chk(logits, "logits"), _stream(feat))
+        return logits
+
+    def predict(self, feat):
+        return self.forward(feat).argmax(dim=1).to(dtype=__import__("torch").int32)
+
+    # ----------------------------------------------------------------- training
+    def grad(self, feat, label, work=None):
+        """(gradient, mean cross-entropy) over the batch."""
+        import torch
+        count = feat.shape[0]
+        if work is None:
+            work = torch.empty(_lib.lib().tsdx_hip_router_grad_workspace_size(count), dtype=torch.uint8,
+                               device=feat.device)
+        g = torch.empty(PARAMS, dtype=torch.float32, device=feat.device)
+        loss = torch.empty(1, dtype=torch.float32, device=feat.device)
+        _lib.call("tsdx_hip_router_grad", _chk(self.params, "params", torch.float32), _chk(feat, "feat", torch.float32),
+                  _chk(label, "label", torch.int32), count, _chk(g, "grad"), _chk(loss, "loss"), _chk(work, "work"),
+                  work.numel(), _stream(feat))
+        return g, loss
+
+    def set_scaling(self, feat):
+        """Feature means and scales from a training set."""
+        mu = feat.mean(dim=0)
+        sd = feat.std(dim=0).clamp_min(1e-3)
+        self.params[0:12] = mu
+        self.params[12:24] = sd
+
+    def fit(self, feat, label, epochs=40, batch=4096, lr=3e-3, seed=0):
+        """Adam on minibatches; returns the mean loss of every epoch."""
+        import torch
+        dev = feat.device
+        self.set_scaling(feat)
+        m = torch.zeros(PARAMS, dtype=torch.float32, device=dev)
+        v = torch.zeros_like(m)
+        count = feat.shape[0]
+        bsz = min(batch, count)
+        work = torch.empty(_lib.lib().tsdx_hip_router_grad_workspace_size(bsz), dtype=torch.uint8, device=dev)
+        gen = torch.Generator(device=dev).manual_seed(seed)
+        step, history = 0, []
+        for _ in range(epochs):
+            perm = torch.randperm(count, generator=gen, device=dev)
+            losses = []
+            for s in range(0, count - bsz + 1, bsz):
+                idx = perm[s:s + bsz]
+                f, l = feat[idx].contiguous(), label[idx].contiguous()
+                g, loss = self.grad(f, l, work)
+                step += 1
+                _lib.call("tsdx_hip_router_adam", _chk(self.params, "params"), _chk(g, "grad"), _chk(m, "m"),
+                          _chk(v, "v"), step, float(lr), _stream(f))
+                losses.append(loss)
+            history.append(float(torch.cat(losses).mean().item()))
+        return history
+
+    def save(self, path):
+        np.save(path, self.params.cpu().numpy())
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        return cls(np.load(path, allow_pickle=False), device=device)
+
+    # -------------------------------------------------------------- routed posv
+    def posv(self, A, B, X, max_iter=10, tol=1e-14, info=None, stat=None, tier=None, work=None):
+        """Routed posv (float64): X := A^-1 B; stat / tier as documented in
+        include/tsdx_hip.h (tsdx_hip_posv_routed_f64)."""
+        import torch
+        batch, n = _shape(A)
+        if work is None:
+            work = torch.empty(_lib.lib().tsdx_hip_posv_routed_workspace_size(batch), dtype=torch.uint8,
+                               device=A.device)
+        _lib.call("tsdx_hip_posv_routed_f64", _chk(A, "A", torch.float64), n, batch, _chk(B, "B", torch.float64),
+                  _chk(X, "X", torch.float64), B.shape[-1], _chk(self.params, "params", torch.float32), int(max_iter),
+                  float(tol), _chk(info, "info", torch.int32), _chk(stat, "stat", torch.int32),
+                  _chk(tier, "tier", torch.int32), _chk(work, "work"), work.numel(), _stream(A))
+        return X, info, stat, tier
