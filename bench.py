@@ -20,6 +20,7 @@ The other workloads of the path (one JSON line each, same fields):
     python bench.py --workload potrf                       fp32 n=32 Cholesky, 65536 matrices
     python bench.py --workload getrf --n 64 --dtype f64    LU + gesv(4 RHS), ~1 GiB of A
     python bench.py --workload ragged --dtype f32          1,000,000 mixed sizes and kinds
+    python bench.py --workload router                      2^20 matrices, kappa up to 1e12, routed vs not
 """
 import argparse
 import json
@@ -41,45 +42,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def haar_q(z):
-    """Orthogonal factor of the QR of z with R's diagonal made positive (the
-    reference's matrices._orthogonal), by classical Gram-Schmidt with one
-    re-orthogonalisation pass (CGS2), batched on the device."""
-    import torch
-    m, n, _ = z.shape
-    q = torch.empty_like(z)
-    for j in range(n):
-        v = z[:, :, j]
-        if j:
-            qj = q[:, :, :j]
-            for _ in range(2):
-                c = (qj * v[:, :, None]).sum(dim=1)
-                v = v - torch.matmul(qj, c[:, :, None])[:, :, 0]
-        q[:, :, j] = v / torch.linalg.vector_norm(v, dim=1, keepdim=True)
-    return q
-
-
 def make_spd(n, batch, kmin, kmax, seed, chunk=16384):
-    """SPD float64 matrices on the device, A = Q diag(lambda) Q^T symmetrised,
-    lambda_0 = 1, lambda_{n-1} = 1/kappa, the rest log-uniform in between,
-    kappa log-uniform on [kmin, kmax] per matrix."""
-    import math
-    import torch
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    A = torch.empty((batch, n, n), dtype=torch.float64, device="cuda")
-    lo, hi = math.log10(kmin), math.log10(kmax)
-    kappa = 10.0 ** (lo + (hi - lo) * torch.rand(batch, generator=g, device="cuda", dtype=torch.float64))
-    for s in range(0, batch, chunk):
-        e = min(batch, s + chunk)
-        m = e - s
-        q = haar_q(torch.randn((m, n, n), generator=g, device="cuda", dtype=torch.float64))
-        t = torch.rand((m, n), generator=g, device="cuda", dtype=torch.float64)
-        t[:, 0] = 0.0
-        t[:, n - 1] = 1.0
-        lam = kappa[s:e, None] ** (-t)
-        a = torch.matmul(q * lam[:, None, :], q.transpose(1, 2).contiguous())
-        A[s:e] = 0.5 * (a + a.transpose(1, 2))
-    return A, kappa
+    """SPD float64 matrices on the device (tinysolverdx.matrices.spd_device):
+    A = Q diag(lambda) Q^T symmetrised, Q Haar, lambda_0 = 1, lambda_{n-1} =
+    1/kappa, the rest log-uniform in between, kappa log-uniform on [kmin, kmax]."""
+    from tinysolverdx import matrices
+    return matrices.spd_device(n, batch, kmin, kmax, seed, chunk=chunk)
 
 
 def backward_error(A, X, B):
@@ -147,10 +115,12 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (reference kernel for scaling rocprof byte counters)")
-    ap.add_argument("--workload", default="posv", choices=["posv", "potrf", "getrf", "ragged"],
+    ap.add_argument("--workload", default="posv", choices=["posv", "potrf", "getrf", "ragged", "router"],
                     help="posv: the headline (default); potrf: fp32 n=32 Cholesky of 65536 matrices; getrf: LU "
                          "+ gesv with 4 right-hand sides on ~1 GiB of A (--n, --dtype); ragged: 1,000,000 "
-                         "matrices of Zipf sizes 2..64, half Cholesky half LU, one graph launch (--dtype)")
+                         "matrices of Zipf sizes 2..64, half Cholesky half LU, one graph launch (--dtype); router: "
+                         "2^20 SPD matrices, n in {16,32,64}, kappa up to 1e12, router trained here, routed vs "
+                         "un-routed posv")
     ap.add_argument("--n", type=int, default=32, help="matrix size of the getrf workload")
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"], help="type of the getrf / ragged workloads")
     a = ap.parse_args(argv)
@@ -158,7 +128,8 @@ def main(argv=None):
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the batched solvers shard by "
               "splitting the batch, with no exchange between GPUs", file=sys.stderr)
         return 2
-    run = {"posv": run_posv, "potrf": run_potrf, "getrf": run_getrf, "ragged": run_ragged}[a.workload]
+    run = {"posv": run_posv, "potrf": run_potrf, "getrf": run_getrf, "ragged": run_ragged,
+           "router": run_router}[a.workload]
     out = run(a)
     print(json.dumps(out), flush=True)
     return 0
@@ -217,27 +188,63 @@ def run_posv(a):
     if ok_m != batch:
         log("[bench] WARNING: not every mixed answer meets the 1e-14 bar")
 
+    # ---- routed: the shipped router (trained on the GPU, tinysolverdx/router_default.npy) picks the tier of
+    # every matrix inside the solve kernel; float64 residual and fallback as above
+    from tinysolverdx import router as rt
+    R = rt.default()
+    Xr = torch.empty_like(B)
+    info_r = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    stat_r = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    tier = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    work_r = torch.empty(tinysolverdx._lib.lib().tsdx_hip_posv_routed_workspace_size(batch), dtype=torch.uint8,
+                         device="cuda")
+
+    def routed():
+        R.posv(A, B, Xr, info=info_r, stat=stat_r, tier=tier, work=work_r)
+
+    wall_r, ev_r = time_steps(routed, a.steps, a.warmup)
+    bw_r = backward_error(A, Xr, B)
+    ok_r = int((bw_r <= 1e-14).sum().item())
+    routed_rate = batch * a.steps / wall_r
+    routed_kern = ev_r / a.steps
+    st_r, tr = stat_r.cpu().numpy(), tier.cpu().numpy()
+    tiers = np.bincount(tr, minlength=3)[:3].tolist()
+    fallback_r = int(((st_r < 0) & (tr != 2)).sum())
+    log(f"[bench] routed posv: {routed_kern * 1e3:.3f} ms/call, {routed_rate / 1e6:.2f} M solves/s, tiers "
+        f"(bf16, fp32, fp64) {tiers}, refinement fallbacks {fallback_r}, bwd err <= 1e-14: {ok_r}/{batch}, "
+        f"max {bw_r.max().item():.2e}")
+    if ok_r != batch:
+        log("[bench] WARNING: not every routed answer meets the 1e-14 bar")
+
     # algorithmic bytes per matrix: read A (8n^2), read B and write X (8n*nrhs each), info (4)
     bytes_per = 8 * n * n + 2 * 8 * n * nrhs + 4
-    achieved = bytes_per * batch / mixed_kern / 1e9
+    achieved = bytes_per * batch / routed_kern / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"k_posv_mx<4> ({a.factor} factor on MFMA + float64 refinement; the event window also "
-                          "holds the fallback-list reset and the gathered float64 launch)",
+                "kernel": "k_posv_mx_routed<4> (features + network + bf16/fp32 factor on MFMA + float64 refinement "
+                          "in one kernel; the event window also holds the fallback-count reset and the gathered "
+                          "float64 launch)",
                 "bytes_per_matrix": bytes_per,
+                "fixed_tier_kernel": {"kernel": f"k_posv_mx<4> ({a.factor})", "ms": round(mixed_kern * 1e3, 4),
+                                      "achieved": round(bytes_per * batch / mixed_kern / 1e9, 1)},
                 "plain_fp64_kernel": {"kernel": "k_posv<double,64,1>", "ms": round(plain_kern * 1e3, 4),
                                       "achieved": round(bytes_per * batch / plain_kern / 1e9, 1)}}
 
     out = {
-        "metric": METRIC, "value": round(mixed_rate, 1), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(wall_m / a.steps * 1e3, 4), "higher_is_better": True,
+        "metric": METRIC, "value": round(routed_rate, 1), "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(wall_r / a.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"posv n={n} SPD kappa log-uniform [1e1,1e6], batch={batch}, nrhs={nrhs}, fp64 inputs",
-                   "tier": f"{a.factor} factor + fp64 refinement (<=10 corrections, tol 1e-14), fp64 fallback",
+                   "tier": "routed: learned router per matrix over {bf16+IR, fp32+IR, fp64}; refinement <=10 "
+                           "corrections to tol 1e-14 with fp64 fallback",
+                   "routed_tiers_bf16_fp32_fp64": tiers, "routed_refinement_fallbacks": fallback_r,
+                   "routed_bwd_err_le_1e-14": f"{ok_r}/{batch}",
+                   "routed_over_plain": round(routed_rate / plain_rate, 3),
+                   "fixed_tier": a.factor, "fixed_tier_solves_per_s": round(mixed_rate, 1),
                    "bwd_err_le_1e-14": f"{ok_m}/{batch}", "fp64_fallback": fallback,
                    "plain_fp64_solves_per_s": round(plain_rate, 1),
                    "plain_fp64_bwd_err_le_1e-14": f"{ok}/{batch}",
-                   "mixed_over_plain": round(mixed_rate / plain_rate, 3)},
+                   "fixed_tier_over_plain": round(mixed_rate / plain_rate, 3)},
         "roofline": roofline,
     }
     if a.copy_probe:
@@ -321,11 +328,12 @@ def make_spd_iid(n, batch, lmin, seed, chunk=16384):
     """A = Q diag(lambda) Q^T symmetrised, lambda i.i.d. log-uniform on [lmin, 1]."""
     import math
     import torch
+    from tinysolverdx import matrices
     g = torch.Generator(device="cuda").manual_seed(seed)
     A = torch.empty((batch, n, n), dtype=torch.float64, device="cuda")
     for s in range(0, batch, chunk):
         e = min(batch, s + chunk)
-        q = haar_q(torch.randn((e - s, n, n), generator=g, device="cuda", dtype=torch.float64))
+        q = matrices.haar_device(torch.randn((e - s, n, n), generator=g, device="cuda", dtype=torch.float64))
         lam = 10.0 ** (math.log10(lmin) * torch.rand((e - s, n), generator=g, device="cuda", dtype=torch.float64))
         a = torch.matmul(q * lam[:, None, :], q.transpose(1, 2).contiguous())
         A[s:e] = 0.5 * (a + a.transpose(1, 2))
@@ -512,6 +520,78 @@ def run_ragged(a):
     return _line(f"ragged factor matrices/s {a.dtype} (1M, n 2..64, half Cholesky half LU); HBM GB/s",
                  count / (ms * 1e-3), "matrices/s", a, ms, a.dtype, cfg,
                  _roof(total, ms * 1e-3, "k_ragged<T,class,variant> x3 (one graph)", round(total / count, 1)), cpu)
+
+
+
+def run_router(a):
+    """Learned router: trained here on separately seeded matrices (labels from
+    the refinement pipeline), then the routed posv against the un-routed fp32
+    refinement and the plain float64 posv on 2^20 SPD matrices, n uniform on
+    {16, 32, 64}, kappa log-uniform on [1, 1e12]; every answer judged at 1e-14."""
+    import numpy as np
+    import torch
+    import tinysolverdx  # noqa: F401
+    from tinysolverdx import device, router as rt
+    total = a.batch if a.batch != 131072 else 1 << 20
+    t0 = time.perf_counter()
+    feat, label = rt.training_set(count_per_n=20000, seed=20261208)
+    R = rt.Router(seed=3)
+    hist = R.fit(feat, label, epochs=60, batch=4096, lr=3e-3, seed=3)
+    train_acc = float((R.predict(feat) == label).float().mean().item())
+    log(f"[bench] router trained in {time.perf_counter() - t0:.1f}s: loss {hist[0]:.3f} -> {hist[-1]:.3f}, "
+        f"labels {np.bincount(label.cpu().numpy(), minlength=3).tolist()}, train accuracy {train_acc:.3f}")
+    sizes = (16, 32, 64)
+    per = total // 3
+    ms = {"routed": 0.0, "unrouted": 0.0, "fp64": 0.0}
+    ok = {"routed": 0, "unrouted": 0, "fp64": 0}
+    tiers = np.zeros(3, np.int64)
+    wasted = {"routed": 0, "unrouted": 0}
+    count = 0
+    for k, n in enumerate(sizes):
+        A, kappa = make_spd(n, per, 1.0, 1e12, seed=20261209 + n)
+        g = torch.Generator(device="cuda").manual_seed(n)
+        B = torch.randn((per, n, 1), generator=g, device="cuda", dtype=torch.float64)
+        X = torch.empty_like(B)
+        info = torch.zeros(per, dtype=torch.int32, device="cuda")
+        stat = torch.zeros(per, dtype=torch.int32, device="cuda")
+        tier = torch.zeros(per, dtype=torch.int32, device="cuda")
+        wr = torch.empty(tinysolverdx._lib.lib().tsdx_hip_posv_routed_workspace_size(per), dtype=torch.uint8,
+                         device="cuda")
+        wi = device.posv_ir_workspace(per)
+        runs = {"routed": lambda: R.posv(A, B, X, info=info, stat=stat, tier=tier, work=wr),
+                "unrouted": lambda: device.posv_ir(A, B, X, factor="fp32", info=info, stat=stat, work=wi),
+                "fp64": lambda: device.posv_solve(A, B, X, info)}
+        for name, fn in runs.items():
+            _, ev = time_steps(fn, a.steps, a.warmup)
+            ms[name] += ev / a.steps * 1e3
+            ok[name] += int((backward_error(A, X, B) <= 1e-14).sum().item())
+            st = stat.cpu().numpy()
+            if name == "routed":
+                tr = tier.cpu().numpy()
+                tiers += np.bincount(tr, minlength=3)[:3]
+                wasted["routed"] += int(((st < 0) & (tr != 2)).sum())
+            elif name == "unrouted":
+                wasted["unrouted"] += int((st < 0).sum())
+        count += per
+        del A, B, X
+        log(f"[bench] router n={n}: cumulative ms {ms}")
+    rate = {k: count / (v * 1e-3) for k, v in ms.items()}
+    es_bytes = sum(8 * n * n + 2 * 8 * n + 4 for n in sizes) * per
+    log(f"[bench] router: routed {rate['routed'] / 1e6:.2f} M/s, un-routed fp32 {rate['unrouted'] / 1e6:.2f} M/s, "
+        f"fp64 {rate['fp64'] / 1e6:.2f} M/s, tiers {tiers.tolist()}, wasted attempts {wasted}")
+    cfg = {"workload": f"routed posv, {count} SPD fp64 matrices, n uniform on {{16,32,64}}, kappa log-uniform "
+                       "[1,1e12], nrhs=1; router trained in this run on 3 x 20000 separately seeded matrices",
+           "train_loss_first_last": [round(hist[0], 4), round(hist[-1], 4)], "train_accuracy": round(train_acc, 4),
+           "train_labels_bf16_fp32_fp64": np.bincount(label.cpu().numpy(), minlength=3).tolist(),
+           "routed_tiers_bf16_fp32_fp64": tiers.tolist(),
+           "unrouted_fp32_solves_per_s": round(rate["unrouted"], 1), "plain_fp64_solves_per_s": round(rate["fp64"], 1),
+           "routed_over_unrouted": round(rate["routed"] / rate["unrouted"], 3),
+           "routed_over_plain_fp64": round(rate["routed"] / rate["fp64"], 3),
+           "wasted_refinement_attempts": wasted, "bwd_err_le_1e-14": {k: f"{v}/{count}" for k, v in ok.items()}}
+    return _line("routed posv solves/s (2^20, n in {16,32,64}, kappa up to 1e12) at bwd err<=1e-14; HBM GB/s",
+                 rate["routed"], "solves/s", a, ms["routed"], "f64", cfg,
+                 _roof(es_bytes, ms["routed"] * 1e-3, "k_posv_mx_routed<1,2,4> + float64 gather",
+                       round(es_bytes / count, 1)), None)
 
 
 if __name__ == "__main__":

@@ -148,14 +148,17 @@ int tsdx_hip_router_grad(const float *params, const float *feat, const int *labe
                          float *loss, void *work, size_t work_bytes, void *stream);
 int tsdx_hip_router_adam(float *params, const float *grad, float *m, float *v, int step, float lr, void *stream);
 
-/* Routed posv (float64 A, B, X): features, network, the refinement tiers over
- * their matrices, then the float64 solve of tier-2 matrices and of every
- * fallback.  stat as for tsdx_hip_posv_ir_f64; tier (batch) or NULL receives
- * the network's choice.  Only the lower triangle of A is read. */
+/* Routed posv (float64 A, B, X).  One kernel reads each matrix once, computes
+ * its features, runs the network and solves in the chosen tier; tier-2
+ * matrices and every refinement that does not converge then take the float64
+ * solve.  stat as for tsdx_hip_posv_ir_f64; tier (batch) and feat (batch, 12)
+ * may be NULL, else they receive the network's choice and the features (the
+ * same definitions as tsdx_hip_router_features_f64).  Only the lower triangle
+ * of A is read. */
 size_t tsdx_hip_posv_routed_workspace_size(int64_t batch);
 int tsdx_hip_posv_routed_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs,
                              const float *params, int max_iter, double tol, int *info, int *stat, int *tier,
-                             void *work, size_t work_bytes, void *stream);
+                             float *feat, void *work, size_t work_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Ragged batches: matrices of different sizes (1..64) and kinds in one call.

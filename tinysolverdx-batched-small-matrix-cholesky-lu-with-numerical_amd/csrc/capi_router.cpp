@@ -21,10 +21,10 @@ void check_count(int64_t count) {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-// routed workspace: [counts (256 B)] [fallback list] [list0] [list1] [features]
+// routed workspace: [fallback count (256 B)] [fallback list]
 size_t routed_workspace(int64_t batch) {
   const size_t b = batch < 0 ? 0 : (size_t)batch;
-  return 256 + 3 * align256(b * sizeof(int)) + align256(b * TSDX_HIP_ROUTER_FEATURES * sizeof(float));
+  return 256 + align256(b * sizeof(int));
 }
 
 }  // namespace
@@ -79,7 +79,7 @@ size_t tsdx_hip_posv_routed_workspace_size(int64_t batch) { return routed_worksp
 
 int tsdx_hip_posv_routed_f64(const double* A, int n, int64_t batch, const double* B, double* X, int nrhs,
                              const float* params, int max_iter, double tol, int* info, int* stat, int* tier,
-                             void* work, size_t work_bytes, void* stream) {
+                             float* feat, void* work, size_t work_bytes, void* stream) {
   return guarded([&] {
     check_dims(n, batch);
     check_count(batch);
@@ -91,12 +91,8 @@ int tsdx_hip_posv_routed_f64(const double* A, int n, int64_t batch, const double
     if (!work || work_bytes < routed_workspace(batch)) throw InvalidArg("workspace missing or too small");
     require_device();
     char* w = static_cast<char*>(work);
-    const size_t li = align256((size_t)batch * sizeof(int));
-    int* counts = reinterpret_cast<int*>(w);  // [0] fallback count, [1] tier 0, [2] tier 1
+    int* fcount = reinterpret_cast<int*>(w);
     int* flist = reinterpret_cast<int*>(w + 256);
-    int* list0 = reinterpret_cast<int*>(w + 256 + li);
-    int* list1 = reinterpret_cast<int*>(w + 256 + 2 * li);
-    float* feat = reinterpret_cast<float*>(w + 256 + 3 * li);
     IrArgs a;
     a.A = A;
     a.B = B;
@@ -109,8 +105,8 @@ int tsdx_hip_posv_routed_f64(const double* A, int n, int64_t batch, const double
     a.info = info;
     a.stat = stat;
     a.flist = flist;
-    a.fcount = counts;
-    hip_check(launch_posv_routed(a, params, feat, tier, list0, list1, counts + 1, S(stream)), "routed posv launch");
+    a.fcount = fcount;
+    hip_check(launch_posv_routed(a, params, tier, feat, S(stream)), "routed posv launch");
   });
 }
 

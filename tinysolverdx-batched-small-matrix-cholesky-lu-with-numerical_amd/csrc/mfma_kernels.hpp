@@ -35,6 +35,7 @@
 #pragma once
 
 #include "mixed_kernels.hpp"
+#include "router_layout.hpp"
 
 namespace tsdx_hip {
 
@@ -211,25 +212,18 @@ __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
 }
 
 // ------------------------------------------------------------------ kernel
-template <int NBLK, Low LF>
-__global__ __launch_bounds__(kWave) void k_posv_mx(IrArgs a) {
-  constexpr int NU = nupper<NBLK>();
-  constexpr int NP = 16 * NBLK;
-  __shared__ __align__(16) double xs[NP];
-  __shared__ __align__(16) float fs[16];
-  const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
-  if (a.list && (int)blockIdx.x >= *a.lcount) return;  // uniform: the whole workgroup leaves
-  const int64_t m = a.list ? a.list[blockIdx.x] : blockIdx.x;
+// Loads the upper blocks of matrix m of a.A (see the file comment) and
+// returns ||A||_F of the symmetric matrix.
+template <int NBLK>
+__device__ __forceinline__ double mx_load(const IrArgs& a, int64_t m, int g, int j, double (&ad)[nupper<NBLK>()][4]) {
   const int n = a.n;
   const double* Am = a.A + m * (int64_t)n * n;
-
   // ---- load the upper blocks of the symmetric matrix (from the lower triangle)
   // Off-diagonal block (I < J): lane (g, j) holds rows 16I + 4g + v, column
   // 16J + j, i.e. A[16J + j][16I + 4g .. 4g + 3] (four consecutive entries of
   // a lower row: two 16-byte loads when rows are 32-byte aligned, vec).
   // Diagonal blocks read A[max(r,c)][min(r,c)] entry by entry.
   const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15) == 0);
-  double ad[NU][4];
   double ss = 0.0;
   static_for<0, NBLK>([&](auto Ic) {
     static_for<Ic, NBLK>([&](auto Jc) {
@@ -261,8 +255,18 @@ __global__ __launch_bounds__(kWave) void k_posv_mx(IrArgs a) {
       }
     });
   });
-  const double anorm = __builtin_sqrt(sum_j(sum_g(ss)));
+  return __builtin_sqrt(sum_j(sum_g(ss)));
+}
 
+// Factor in the low format LF, refine every right-hand side, and record the
+// outcome: X written for converged matrices, the others appended to the
+// float64 fallback list; info 0 and stat (corrections, or -1) for all.
+template <int NBLK, Low LF>
+__device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int j,
+                                         const double (&ad)[nupper<NBLK>()][4], double anorm, double* xs, float* fs) {
+  constexpr int NU = nupper<NBLK>();
+  const int lane = threadIdx.x;
+  const int n = a.n;
   // ---- factor: float copy, block right-looking, MFMA trailing updates
   v4f w[NU];
   static_for<0, NU>([&](auto b) {
@@ -408,6 +412,164 @@ __global__ __launch_bounds__(kWave) void k_posv_mx(IrArgs a) {
     if (a.info) a.info[m] = 0;
     if (a.stat) a.stat[m] = ok ? used : -1;
   }
+}
+
+template <int NBLK, Low LF>
+__global__ __launch_bounds__(kWave) void k_posv_mx(IrArgs a) {
+  __shared__ __align__(16) double xs[16 * NBLK];
+  __shared__ __align__(16) float fs[16];
+  const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
+  if (a.list && (int)blockIdx.x >= *a.lcount) return;  // uniform: the whole workgroup leaves
+  const int64_t m = a.list ? a.list[blockIdx.x] : blockIdx.x;
+  double ad[nupper<NBLK>()][4];
+  const double anorm = mx_load<NBLK>(a, m, g, j, ad);
+  mx_solve<NBLK, LF>(a, m, g, j, ad, anorm, xs, fs);
+}
+
+// ------------------------------------------------------------ in-wave router
+// The 12 router features (definitions in router_kernels.hpp, k_router_features)
+// of the symmetric matrix held in ad; every lane gets every feature.
+template <int NBLK>
+__device__ __forceinline__ void mx_features(const double (&ad)[nupper<NBLK>()][4], int n, int g, int j, double anorm,
+                                            float (&f)[kRF]) {
+  const int src_mixed = 16 * (j >> 2) + 4 * (j & 3);
+  double rabs[NBLK], dg[NBLK], pc[NBLK];
+  static_for<0, NBLK>([&](auto J) { pc[J] = 0.0; });
+  static_for<0, NBLK>([&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    double p[4] = {0.0, 0.0, 0.0, 0.0};
+    static_for<I, NBLK>([&](auto Jc) {
+      constexpr int J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
+      static_for<0, 4>([&](auto v) {
+        p[v] += __builtin_fabs(ad[b][v()]);
+        if constexpr (I < J) pc[J] += __builtin_fabs(ad[b][v()]);
+      });
+    });
+    rabs[I] = bperm(tsum4(p, j), src_mixed);  // row sums of |a| through the upper blocks, column form
+    double dv = 0.0;
+    static_for<0, 4>([&](auto v) { dv = (4 * g + v == j) ? ad[bslot<NBLK>(I, I)][v()] : dv; });
+    dg[I] = sum_g(dv);  // diagonal, column form
+  });
+  const float big = 3.0e38f;
+  float dmin = big, dmax = 0.0f, ninf = 0.0f, gers = big, dom = 0.0f, dsum = 0.0f, lsum = 0.0f, nonpos = 0.0f,
+        ddom = 0.0f;
+  static_for<0, NBLK>([&](auto J) {
+    const bool act = 16 * J + j < n;
+    const double ra = rabs[J] + sum_g(pc[J]);
+    const float d = (float)dg[J], r = (float)(ra - __builtin_fabs(dg[J]));
+    if (act) {
+      dmin = d < dmin ? d : dmin;
+      dmax = __builtin_fabsf(d) > dmax ? __builtin_fabsf(d) : dmax;
+      ninf = (float)ra > ninf ? (float)ra : ninf;
+      gers = d - r < gers ? d - r : gers;
+      const float q = d > 0.0f ? r / d : big;
+      dom = q > dom ? q : dom;
+      dsum += d;
+      lsum += flog10(d);
+      nonpos = (d > 0.0f) ? nonpos : 1.0f;
+      ddom += (d > r) ? 1.0f : 0.0f;
+    }
+  });
+  auto red = [](float x, auto op) {
+    x = op(x, xshfl<1>(x));
+    x = op(x, xshfl<2>(x));
+    x = op(x, xshfl<7>(x));
+    return op(x, xshfl<15>(x));
+  };
+  auto fsum = [](float x, float y) { return x + y; };
+  auto fmin = [](float x, float y) { return x < y ? x : y; };
+  auto fmax = [](float x, float y) { return x > y ? x : y; };
+  dmin = red(dmin, fmin);
+  dmax = red(dmax, fmax);
+  ninf = red(ninf, fmax);
+  gers = red(gers, fmin);
+  dom = red(dom, fmax);
+  dsum = red(dsum, fsum);
+  lsum = red(lsum, fsum);
+  nonpos = red(nonpos, fmax);
+  ddom = red(ddom, fsum);
+  const float fr = (float)anorm;
+  f[0] = (float)n / 64.0f;
+  f[1] = flog10(dmin);
+  f[2] = flog10(dmax);
+  f[3] = flog10(fr);
+  f[4] = flog10(ninf);
+  f[5] = fr > 0.0f ? gers / fr : 0.0f;
+  f[5] = f[5] < -30.0f ? -30.0f : f[5];
+  f[6] = dom >= big ? 30.0f : flog10(dom);
+  f[7] = dmin > 0.0f ? flog10(dmax / dmin) : 30.0f;
+  f[8] = dsum > 0.0f ? flog10(fr / (dsum / (float)n)) : 30.0f;
+  f[9] = lsum / (float)n - flog10(dmax);
+  f[10] = nonpos;
+  f[11] = ddom / (float)n;
+}
+
+// The network on one wave (lane o = hidden unit o); returns the tier, the
+// first largest logit (a NaN logit gives tier 2).  hs: 64 floats of LDS.
+__device__ __forceinline__ int mx_route(const float* __restrict__ P, const float (&f)[kRF], int lane, float* hs) {
+  float x[kRF];
+  static_for<0, kRF>([&](auto q) { x[q] = (f[q] - P[q]) / P[kRF + q]; });
+  float a1 = P[kOffB1 + lane];
+  static_for<0, kRF>([&](auto q) { a1 = __builtin_fmaf(P[kOffW1 + lane * kRF + q], x[q], a1); });
+  hs[lane] = a1 > 0.0f ? a1 : 0.0f;
+  wave_sync();
+  float a2 = P[kOffB2 + lane];
+  const float4* w2 = reinterpret_cast<const float4*>(P + kOffW2 + lane * kRH);  // kOffW2 is a multiple of 4
+  static_for<0, kRH / 4>([&](auto q) {
+    const float4 w = w2[q];
+    const float4 h = *reinterpret_cast<const float4*>(hs + 4 * q);
+    a2 = __builtin_fmaf(w.x, h.x, a2);
+    a2 = __builtin_fmaf(w.y, h.y, a2);
+    a2 = __builtin_fmaf(w.z, h.z, a2);
+    a2 = __builtin_fmaf(w.w, h.w, a2);
+  });
+  const float h2 = a2 > 0.0f ? a2 : 0.0f;
+  float z[kRT];
+  static_for<0, kRT>([&](auto t) {
+    z[t] = group_allreduce<64>(P[kOffW3 + t * kRH + lane] * h2, [](float u, float v) { return u + v; }) +
+           P[kOffB3 + t];
+  });
+  int t = 0;
+  t = z[1] > z[0] ? 1 : t;
+  t = z[2] > (t == 1 ? z[1] : z[0]) ? 2 : t;
+  if (!(z[0] == z[0] && z[1] == z[1] && z[2] == z[2])) t = 2;
+  return t;
+}
+
+// Routed posv: features and network in the wave that then solves the matrix
+// (A is read from HBM once): tier 0 / 1 factor in bf16 / fp32 and refine,
+// tier 2 goes straight to the float64 list (stat -1).  tier_out / feat_out
+// (optional) receive the choice and the features.
+template <int NBLK>
+__global__ __launch_bounds__(kWave) void k_posv_mx_routed(IrArgs a, const float* __restrict__ P,
+                                                          int* __restrict__ tier_out, float* __restrict__ feat_out) {
+  __shared__ __align__(16) double xs[16 * NBLK];
+  __shared__ __align__(16) float fs[16];
+  __shared__ __align__(16) float hs[kRH];
+  const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
+  const int64_t m = blockIdx.x;
+  double ad[nupper<NBLK>()][4];
+  const double anorm = mx_load<NBLK>(a, m, g, j, ad);
+  float f[kRF];
+  mx_features<NBLK>(ad, a.n, g, j, anorm, f);
+  if (feat_out && lane < kRF) {
+    float v = f[0];
+    static_for<1, kRF>([&](auto q) { v = (lane == q) ? f[q] : v; });
+    feat_out[m * kRF + lane] = v;
+  }
+  const int t = mx_route(P, f, lane, hs);
+  if (tier_out && lane == 0) tier_out[m] = t;
+  if (t == 2) {
+    if (lane == 0) {
+      a.flist[atomicAdd(a.fcount, 1)] = (int)m;
+      if (a.stat) a.stat[m] = -1;
+    }
+    return;
+  }
+  if (t == 0)
+    mx_solve<NBLK, Low::BF16>(a, m, g, j, ad, anorm, xs, fs);
+  else
+    mx_solve<NBLK, Low::F32>(a, m, g, j, ad, anorm, xs, fs);
 }
 
 }  // namespace tsdx_hip

@@ -45,31 +45,20 @@ __global__ void k_zero_counts(int* p, int k) {
   if ((int)threadIdx.x < k) p[threadIdx.x] = 0;
 }
 
-template <int NBLK>
-static void routed_mx(IrArgs a, const int* list0, const int* c0, const int* list1, const int* c1, hipStream_t s) {
-  const dim3 grid((unsigned)a.batch), block(kWave);
-  a.list = list0;
-  a.lcount = c0;
-  hipLaunchKernelGGL((k_posv_mx<NBLK, Low::BF16>), grid, block, 0, s, a);
-  a.list = list1;
-  a.lcount = c1;
-  hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F32>), grid, block, 0, s, a);
-}
-
-hipError_t launch_posv_routed(const IrArgs& a, const float* P, float* feat, int* tier, int* list0, int* list1,
-                              int* counts, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, counts, 2);
+hipError_t launch_posv_routed(const IrArgs& a, const float* P, int* tier, float* feat, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, a.fcount, 1);
-  hipError_t e = launch_router_features(a.A, a.n, a.batch, feat, s);
-  if (e != hipSuccess) return e;
-  const int64_t blocks = (a.batch + kMlpThreads - 1) / kMlpThreads;
-  hipLaunchKernelGGL(k_router_route, dim3((unsigned)blocks), dim3(kMlpThreads), 0, s, P, feat, a.batch, tier, list0,
-                     list1, counts, a.flist, a.fcount, a.stat);
+  const dim3 grid((unsigned)a.batch), block(kWave);
   const int nblk = (a.n + 15) / 16;
-  if (nblk == 1) routed_mx<1>(a, list0, counts, list1, counts + 1, s);
-  else if (nblk == 2) routed_mx<2>(a, list0, counts, list1, counts + 1, s);
-  else if (nblk == 3) routed_mx<3>(a, list0, counts, list1, counts + 1, s);
-  else routed_mx<4>(a, list0, counts, list1, counts + 1, s);
+  if (nblk == 1)
+    hipLaunchKernelGGL((k_posv_mx_routed<1>), grid, block, 0, s, a, P, tier, feat);
+  else if (nblk == 2)
+    hipLaunchKernelGGL((k_posv_mx_routed<2>), grid, block, 0, s, a, P, tier, feat);
+  else if (nblk == 3)
+    hipLaunchKernelGGL((k_posv_mx_routed<3>), grid, block, 0, s, a, P, tier, feat);
+  else
+    hipLaunchKernelGGL((k_posv_mx_routed<4>), grid, block, 0, s, a, P, tier, feat);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   return launch_posv_gather(a, s);
 }
 

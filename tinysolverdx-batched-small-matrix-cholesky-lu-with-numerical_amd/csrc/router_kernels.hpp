@@ -12,13 +12,9 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "router_layout.hpp"
 
 namespace tsdx_hip {
-
-constexpr int kRF = 12, kRH = 64, kRT = 3;
-constexpr int kOffW1 = 24, kOffB1 = kOffW1 + kRH * kRF, kOffW2 = kOffB1 + kRH, kOffB2 = kOffW2 + kRH * kRH,
-              kOffW3 = kOffB2 + kRH, kOffB3 = kOffW3 + kRT * kRH, kRouterParams = kOffB3 + kRT;
-static_assert(kRouterParams == 5211, "parameter layout");
 
 // ------------------------------------------------------------------ features
 // One wave per matrix; the symmetric matrix of A's lower triangle, in float.
@@ -32,8 +28,6 @@ static_assert(kRouterParams == 5211, "parameter layout");
 // non-positive diagonal to +30).
 constexpr int kFeatWPB = 2;  // 2 x 16.6 KB of LDS
 constexpr int kFeatLd = 65;  // padded row stride of the LDS copy
-
-__device__ __forceinline__ float flog10(float x) { return x > 1e-30f ? __log10f(x) : -30.0f; }
 
 __global__ __launch_bounds__(kWave* kFeatWPB) void k_router_features(const double* __restrict__ A, int n,
                                                                      int64_t batch, float* __restrict__ feat) {
@@ -146,34 +140,6 @@ __global__ __launch_bounds__(kMlpThreads) void k_router_forward(const float* __r
   MlpState st;
   mlp_forward(sP, feat + s * kRF, st);
   for (int o = 0; o < kRT; ++o) logits[s * kRT + o] = st.z[o];
-}
-
-// Routing: tier = argmax of the logits (first maximum); tier 0 / 1 matrices
-// are appended to their lists, tier 2 to the float64 list (stat = -1).
-__global__ __launch_bounds__(kMlpThreads) void k_router_route(const float* __restrict__ P, const float* __restrict__ feat,
-                                                              int64_t batch, int* __restrict__ tier, int* __restrict__ list0,
-                                                              int* __restrict__ list1, int* __restrict__ counts,
-                                                              int* __restrict__ flist, int* __restrict__ fcount,
-                                                              int* __restrict__ stat) {
-  __shared__ float sP[kRouterParams];
-  load_params(sP, P);
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= batch) return;
-  MlpState st;
-  mlp_forward(sP, feat + s * kRF, st);
-  int t = 0;
-  if (st.z[1] > st.z[t]) t = 1;
-  if (st.z[2] > st.z[t]) t = 2;
-  if (!(st.z[0] == st.z[0] && st.z[1] == st.z[1] && st.z[2] == st.z[2])) t = 2;  // NaN features: float64
-  if (tier) tier[s] = t;
-  if (t == 0) {
-    list0[atomicAdd(counts + 0, 1)] = (int)s;
-  } else if (t == 1) {
-    list1[atomicAdd(counts + 1, 1)] = (int)s;
-  } else {
-    flist[atomicAdd(fcount, 1)] = (int)s;
-    if (stat) stat[s] = -1;
-  }
 }
 
 // ------------------------------------------------------------------ training

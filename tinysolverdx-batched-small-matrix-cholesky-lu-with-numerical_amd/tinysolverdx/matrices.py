@@ -46,3 +46,45 @@ def general_kappas(rng, kappas, n):
     u = orthogonal(rng, len(s), n)
     v = orthogonal(rng, len(s), n)
     return np.einsum("bij,bj,bkj->bik", u, s, v)
+
+
+# ----------------------------------------------------------- on the device
+def haar_device(z):
+    """Orthogonal factor of the QR of z (batch, n, n), R's diagonal made
+    positive, by classical Gram-Schmidt with one re-orthogonalisation pass
+    (CGS2), batched with torch on z's device."""
+    import torch
+    m, n, _ = z.shape
+    q = torch.empty_like(z)
+    for j in range(n):
+        v = z[:, :, j]
+        if j:
+            qj = q[:, :, :j]
+            for _ in range(2):
+                c = (qj * v[:, :, None]).sum(dim=1)
+                v = v - torch.matmul(qj, c[:, :, None])[:, :, 0]
+        q[:, :, j] = v / torch.linalg.vector_norm(v, dim=1, keepdim=True)
+    return q
+
+
+def spd_device(n, batch, kmin, kmax, seed, device="cuda", chunk=16384):
+    """float64 SPD matrices on the device: Q diag(lambda) Q^T symmetrised,
+    lambda_0 = 1, lambda_{n-1} = 1/kappa, the rest log-uniform in between,
+    kappa log-uniform on [kmin, kmax] per matrix.  Returns (A, kappa)."""
+    import math
+    import torch
+    g = torch.Generator(device=device).manual_seed(seed)
+    A = torch.empty((batch, n, n), dtype=torch.float64, device=device)
+    lo, hi = math.log10(kmin), math.log10(kmax)
+    kappa = 10.0 ** (lo + (hi - lo) * torch.rand(batch, generator=g, device=device, dtype=torch.float64))
+    for s in range(0, batch, chunk):
+        e = min(batch, s + chunk)
+        m = e - s
+        q = haar_device(torch.randn((m, n, n), generator=g, device=device, dtype=torch.float64))
+        t = torch.rand((m, n), generator=g, device=device, dtype=torch.float64)
+        t[:, 0] = 0.0
+        t[:, n - 1] = 1.0
+        lam = kappa[s:e, None] ** (-t)
+        a = torch.matmul(q * lam[:, None, :], q.transpose(1, 2).contiguous())
+        A[s:e] = 0.5 * (a + a.transpose(1, 2))
+    return A, kappa

@@ -8,8 +8,11 @@ SPD matrix, the cheapest posv tier expected to reach the tolerance:
 from 12 features taken in one pass over the lower triangle of A.  Features,
 forward pass, gradient and Adam step are the library's HIP kernels
 (include/tsdx_hip.h, tsdx_hip_router_*); torch only holds the buffers and
-shuffles minibatches.  Labels come from the refinement pipeline itself
-(make_labels): the cheapest tier whose refinement converged on that matrix.
+shuffles minibatches.  Labels come from the pipeline itself (make_labels):
+every tier runs on the training matrices, and a matrix's label is the tier
+with the smallest cost for it, costs measured per size at throughput
+(measure_costs; a refinement that does not converge pays for the float64
+solve as well).
 
 The routed solve (Router.posv) never trusts the network for accuracy: every
 answer is judged by the float64 residual, refinement that stalls falls back
@@ -40,23 +43,103 @@ def features(A):
     return feat
 
 
-def make_labels(A, B, max_iter=10, tol=1e-14):
-    """Cheapest converging tier per matrix: 0 if the bf16 refinement converges,
-    else 1 if the fp32 refinement does, else 2 (float64)."""
+def _timed(fn, reps=3):
+    """Smallest HIP-event time (seconds) of fn over reps runs on torch's stream."""
+    import torch
+    best = float("inf")
+    s = torch.cuda.current_stream()
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    return best
+
+
+def measure_costs(n, count=32768, max_iter=10, tol=1e-14, seed=1):
+    """Time per matrix (seconds) of each tier at size n, at throughput: the
+    refinement tiers on well-conditioned matrices (kappa up to 1e3, so they
+    converge and the float64 fallback stays empty), the float64 tier with the
+    plain posv.  Returns {0: bf16, 1: fp32, 2: fp64}."""
+    import torch
+    from . import device, matrices
+    A, _ = matrices.spd_device(n, count, 1.0, 1e3, seed)
+    B = torch.randn((count, n, 1), generator=torch.Generator(device="cuda").manual_seed(seed), device="cuda",
+                    dtype=torch.float64)
+    X = torch.empty_like(B)
+    info = torch.empty(count, dtype=torch.int32, device="cuda")
+    work = device.posv_ir_workspace(count)
+    cost = {t: _timed(lambda: device.posv_ir(A, B, X, factor=f, max_iter=max_iter, tol=tol, work=work)) / count
+            for t, f in ((0, "bf16"), (1, "fp32"))}
+    cost[2] = _timed(lambda: device.posv_solve(A, B, X, info)) / count
+    return cost
+
+
+def make_labels(A, B, cost, max_iter=10, tol=1e-14):
+    """The cheapest tier for each matrix of one size under the per-matrix tier
+    costs `cost` ({tier: seconds}, from measure_costs at that size): a
+    refinement tier costs cost[t] when it converges on the matrix (its stat)
+    and cost[t] + cost[2] when the float64 fallback has to follow; ties go to
+    the lower precision."""
     import torch
     from . import device
     batch = A.shape[0]
     X = torch.empty_like(B)
     work = device.posv_ir_workspace(batch, A.device)
-    st = {}
-    for f in ("bf16", "fp32"):
+    exp = []
+    for t, f in ((0, "bf16"), (1, "fp32")):
         s = torch.empty(batch, dtype=torch.int32, device=A.device)
         device.posv_ir(A, B, X, factor=f, max_iter=max_iter, tol=tol, stat=s, work=work)
-        st[f] = s
-    label = torch.full((batch,), 2, dtype=torch.int32, device=A.device)
-    label[st["fp32"] >= 0] = 1
-    label[st["bf16"] >= 0] = 0
-    return label
+        exp.append(torch.where(s >= 0, torch.tensor(cost[t], dtype=torch.float64, device=A.device),
+                               torch.tensor(cost[t] + cost[2], dtype=torch.float64, device=A.device)))
+    exp.append(torch.full((batch,), cost[2], device=A.device, dtype=torch.float64))
+    return torch.stack(exp, dim=1).argmin(dim=1).to(torch.int32)  # first minimum: lower precision on ties
+
+
+DEFAULT_PARAMS = __import__("pathlib").Path(__file__).resolve().parent / "router_default.npy"
+
+
+def training_set(count_per_n=20000, sizes=(16, 32, 64), kmax=1e12, seed=20261207, device="cuda"):
+    """Features and labels of SPD matrices with kappa log-uniform on [1, kmax]
+    (one block per size), generated on the host from `seed`."""
+    import torch
+    from . import matrices
+    rng = np.random.default_rng(seed)
+    feats, labels = [], []
+    costs = {}
+    for n in sizes:
+        kappa = 10.0 ** rng.uniform(0.0, np.log10(kmax), count_per_n)
+        A = torch.from_numpy(matrices.spd_kappas(rng, kappa, n)).to(device)
+        B = torch.from_numpy(rng.standard_normal((count_per_n, n, 1))).to(device)
+        costs[n] = measure_costs(n)
+        labels.append(make_labels(A, B, costs[n]))
+        feats.append(features(A))
+        del A, B
+    training_set.costs = costs
+    return torch.cat(feats), torch.cat(labels)
+
+
+def train_default(path=DEFAULT_PARAMS, count_per_n=20000, epochs=60, seed=20261207):
+    """Trains the router shipped with the package (kappa up to 1e12, n in
+    {16, 32, 64}) with the library's kernels and writes its parameters."""
+    feat, label = training_set(count_per_n, seed=seed)
+    r = Router(seed=seed)
+    hist = r.fit(feat, label, epochs=epochs, batch=4096, lr=3e-3, seed=seed)
+    r.save(path)
+    counts = np.bincount(label.cpu().numpy(), minlength=3).tolist()
+    acc = float((r.predict(feat) == label).float().mean().item())
+    costs = {n: {t: round(c * 1e9, 3) for t, c in v.items()} for n, v in training_set.costs.items()}
+    return {"loss_first": hist[0], "loss_last": hist[-1], "label_counts": counts, "train_accuracy": acc,
+            "cost_ns_per_matrix": costs}
+
+
+def default():
+    """The router trained by train_default() and shipped with the package."""
+    if not DEFAULT_PARAMS.exists():
+        raise RuntimeError(f"{DEFAULT_PARAMS} is missing: run python -m tinysolverdx.router --train")
+    return Router.load(DEFAULT_PARAMS)
 
 
 def init_params(seed=0):
@@ -149,9 +232,9 @@ class Router:
         return cls(np.load(path, allow_pickle=False), device=device)
 
     # -------------------------------------------------------------- routed posv
-    def posv(self, A, B, X, max_iter=10, tol=1e-14, info=None, stat=None, tier=None, work=None):
-        """Routed posv (float64): X := A^-1 B; stat / tier as documented in
-        include/tsdx_hip.h (tsdx_hip_posv_routed_f64)."""
+    def posv(self, A, B, X, max_iter=10, tol=1e-14, info=None, stat=None, tier=None, feat=None, work=None):
+        """Routed posv (float64): X := A^-1 B; stat / tier / feat as documented
+        in include/tsdx_hip.h (tsdx_hip_posv_routed_f64)."""
         import torch
         batch, n = _shape(A)
         if work is None:
@@ -160,5 +243,19 @@ class Router:
         _lib.call("tsdx_hip_posv_routed_f64", _chk(A, "A", torch.float64), n, batch, _chk(B, "B", torch.float64),
                   _chk(X, "X", torch.float64), B.shape[-1], _chk(self.params, "params", torch.float32), int(max_iter),
                   float(tol), _chk(info, "info", torch.int32), _chk(stat, "stat", torch.int32),
-                  _chk(tier, "tier", torch.int32), _chk(work, "work"), work.numel(), _stream(A))
+                  _chk(tier, "tier", torch.int32), _chk(feat, "feat", torch.float32), _chk(work, "work"),
+                  work.numel(), _stream(A))
         return X, info, stat, tier
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="train the default precision router on the GPU")
+    ap.add_argument("--train", action="store_true")
+    ap.add_argument("--out", default=str(DEFAULT_PARAMS))
+    ap.add_argument("--count-per-n", type=int, default=20000)
+    ap.add_argument("--epochs", type=int, default=60)
+    args = ap.parse_args()
+    if args.train:
+        print(json.dumps(train_default(args.out, args.count_per_n, args.epochs)), flush=True)
