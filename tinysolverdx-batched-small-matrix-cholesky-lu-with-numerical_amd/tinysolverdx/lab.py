@@ -1,0 +1,191 @@
+"""GPU error lab: backward / forward error and failure rate against condition
+number and size, for every result this library produces on the GPU.
+
+The structure follows the reference's numerical lab (python/tinysolverdx/lab.py:36-178):
+matrices with a prescribed 2-norm condition number (tinysolverdx.matrices), the
+same inputs for every precision, judged against the long-double ground truth of
+the host API (reference_solve, chol/lu/solve backward errors, solve forward
+error; runtime.cpp:162-260 in the reference).  What is measured here is what
+the GPU computes:
+
+* plain fp32 / fp64 Cholesky (posv) and LU (gesv) with backend="hip"
+  (the reference's CPU emulation of fp16 / bf16 has no GPU path);
+* the posv tiers: fp32 / bf16 / fp16 factor + float64 refinement with the
+  float64 fallback, and the routed solve (the shipped router), for which the
+  "failure" is the fraction of matrices that needed the float64 solve.
+
+    python -m tinysolverdx.lab --out DIR [--quick]
+
+writes error_vs_cond.csv, error_vs_n.csv, tiers_vs_cond.csv and REPORT.md.
+"""
+import argparse
+import csv
+import os
+import pathlib
+
+import numpy as np
+
+import tinysolverdx as tsdx
+from tinysolverdx import matrices
+
+PLAIN = ["fp32", "fp64"]
+TIERS = ["fp32+IR", "bf16+IR", "fp16+IR", "routed"]
+
+
+def _med(v, ok):
+    v = v[ok]
+    return float(np.median(v)) if v.size else np.nan
+
+
+def _max(v, ok):
+    v = v[ok]
+    return float(np.max(v)) if v.size else np.nan
+
+
+def run_plain_point(rng, prec, kappa, n, trials):
+    """One row per kind for (precision, kappa, n): factor and solve on the GPU."""
+    rows = []
+    b = rng.standard_normal((trials, n))
+    for kind, gen in (("chol", matrices.spd), ("lu", matrices.general)):
+        A = tsdx.round_to(gen(rng, trials, n, kappa), prec)  # the matrix actually factored
+        cond = np.linalg.cond(A)
+        if kind == "chol":
+            L, info = tsdx.cholesky(A, precision=prec, backend="hip")
+            bw = tsdx.chol_backward_error(A, L.astype(np.float64))
+            x, _ = tsdx.posv(A, b, precision=prec, backend="hip")
+        else:
+            LU, piv, info = tsdx.lu(A, precision=prec, backend="hip")
+            bw = tsdx.lu_backward_error(A, LU.astype(np.float64), piv)
+            x, _ = tsdx.gesv(A, b, precision=prec, backend="hip")
+        ok = (info == 0) & np.isfinite(bw)
+        x = x.astype(np.float64)
+        okx = ok & np.isfinite(x).all(axis=1)
+        x0 = np.where(okx[:, None], x, 0.0)
+        fwd = tsdx.solve_forward_error(A, x0, b)
+        sbw = tsdx.solve_backward_error(A, x0, b)
+        rows.append(dict(precision=prec, kind=kind, n=n, kappa=kappa, cond_actual=_med(cond, np.isfinite(cond)),
+                         fail_frac=float(1 - ok.mean()), backward_med=_med(bw, ok), backward_max=_max(bw, ok),
+                         solve_backward_med=_med(sbw, okx), forward_med=_med(fwd, okx), forward_max=_max(fwd, okx)))
+    return rows
+
+
+def run_tier_point(rng, tier, kappa, n, trials):
+    """One row for a posv tier at (kappa, n): float64 A and b on the GPU."""
+    import torch
+    from tinysolverdx import device, router
+    A = matrices.spd(rng, trials, n, kappa)
+    b = rng.standard_normal((trials, n))
+    dA = torch.from_numpy(A).cuda()
+    dB = torch.from_numpy(b[:, :, None].copy()).cuda()
+    X = torch.zeros_like(dB)
+    stat = torch.zeros(trials, dtype=torch.int32, device="cuda")
+    info = torch.zeros(trials, dtype=torch.int32, device="cuda")
+    if tier == "routed":
+        router.default().posv(dA, dB, X, info=info, stat=stat)
+    else:
+        device.posv_ir(dA, dB, X, factor=tier.split("+")[0], info=info, stat=stat)
+    torch.cuda.synchronize()
+    x = X.cpu().numpy()[:, :, 0]
+    st, inf = stat.cpu().numpy(), info.cpu().numpy()
+    ok = inf == 0
+    sbw = tsdx.solve_backward_error(A, x, b)
+    fwd = tsdx.solve_forward_error(A, x, b)
+    return dict(tier=tier, n=n, kappa=kappa, fallback_frac=float((st < 0).mean()),
+                corrections_mean=float(st[st >= 0].mean()) if (st >= 0).any() else np.nan,
+                solve_backward_med=_med(sbw, ok), solve_backward_max=_max(sbw, ok),
+                forward_med=_med(fwd, ok), forward_max=_max(fwd, ok), info_gt0=int((~ok).sum()))
+
+
+def write_csv(path, rows):
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=list(rows[0].keys()))
+        w.writeheader()
+        w.writerows(rows)
+
+
+def fmt(v):
+    return "-" if v is None or (isinstance(v, float) and np.isnan(v)) else f"{v:.2e}"
+
+
+def write_report(path, cond_rows, size_rows, tier_rows, n_fixed, n_tier, size_kappa, trials, seed):
+    L = ["# tinysolverdx-hip numerical error report (MI355X)", "",
+         f"Generated by `python -m tinysolverdx.lab` (seed={seed}, {trials} matrices per point).  Matrices with a "
+         "prescribed 2-norm condition number (Haar Q, geometric spectrum), rounded to the working precision, "
+         "factored and solved on the GPU, and judged on the host against a long-double reference "
+         "(`reference_solve`, long-double residuals).", "",
+         "Metrics: backward error `||A-LL^T||_F/||A||_F` (Cholesky) and `||PA-LU||_F/||A||_F` (LU); solve backward "
+         "error `||b-Ax||_2/(||A||_F||x||_2+||b||_2)`; solve forward error `||x-x_ref||_2/||x_ref||_2`; failure rate = "
+         "fraction with `info>0` (plain) or fraction that needed the float64 solve (refinement tiers).  Medians are "
+         "over successful factorizations.", ""]
+    for prec in PLAIN:
+        L += [f"## plain {prec} on the GPU, n={n_fixed} (u = {tsdx.unit_roundoff(prec):.2e})", "",
+              "| kappa | chol backward | chol forward | chol fail | LU backward | LU forward | LU fail |",
+              "|---|---|---|---|---|---|---|"]
+        for k in sorted({r["kappa"] for r in cond_rows}):
+            c = next(r for r in cond_rows if r["precision"] == prec and r["kind"] == "chol" and r["kappa"] == k)
+            g = next(r for r in cond_rows if r["precision"] == prec and r["kind"] == "lu" and r["kappa"] == k)
+            L.append(f"| {k:.0e} | {fmt(c['backward_med'])} | {fmt(c['forward_med'])} | {c['fail_frac']:.2f} | "
+                     f"{fmt(g['backward_med'])} | {fmt(g['forward_med'])} | {g['fail_frac']:.2f} |")
+        L.append("")
+    L += [f"## size sweep, kappa={size_kappa:g}", "", "| n | precision | chol backward | LU backward | chol forward | LU forward |",
+          "|---|---|---|---|---|---|"]
+    for r in size_rows:
+        if r["kind"] != "chol":
+            continue
+        g = next(q for q in size_rows if q["kind"] == "lu" and q["n"] == r["n"] and q["precision"] == r["precision"])
+        L.append(f"| {r['n']} | {r['precision']} | {fmt(r['backward_med'])} | {fmt(g['backward_med'])} | "
+                 f"{fmt(r['forward_med'])} | {fmt(g['forward_med'])} |")
+    L.append("")
+    L += [f"## posv tiers (float64 A and b), n={n_tier}", "",
+          "Every answer of a tier is either a refinement that reached ||b-Ax||/(||A|| ||x||+||b||) <= 5e-15 or the "
+          "float64 solve.", "",
+          "| kappa | tier | backward (max) | forward (median) | to float64 | corrections (mean) |", "|---|---|---|---|---|---|"]
+    for k in sorted({r["kappa"] for r in tier_rows}):
+        for t in TIERS:
+            r = next(q for q in tier_rows if q["kappa"] == k and q["tier"] == t)
+            L.append(f"| {k:.0e} | {t} | {fmt(r['solve_backward_max'])} | {fmt(r['forward_med'])} | "
+                     f"{r['fallback_frac']:.2f} | {fmt(r['corrections_mean'])} |")
+    L += ["", "## Reading the results", "",
+          "* The plain GPU factorizations are bit-identical to the reference's CPU Auto backend, so their backward "
+          "errors are the reference's: a small multiple of u, flat in kappa while the factorization succeeds.",
+          "* Forward error tracks kappa*u; Cholesky starts failing (info>0) once kappa*u approaches 1.",
+          "* The refinement tiers keep the backward error at the float64 level (<= 5e-15 by construction) for every "
+          "kappa; what changes with kappa is the number of corrections and the fraction that needs the float64 "
+          "solve: bf16 / fp16 factors stop converging near kappa ~ 1/u_low, fp32 near 1e7.", ""]
+    path.write_text("\n".join(L))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", default=os.environ.get("TSDX_OUT_DIR", "lab_report"))
+    ap.add_argument("--seed", type=int, default=int(os.environ.get("TSDX_SEED", "0")))
+    ap.add_argument("--trials", type=int, default=100)
+    ap.add_argument("--n", type=int, default=16, help="matrix size for the condition sweep of the plain kernels")
+    ap.add_argument("--n-tier", type=int, default=64, help="matrix size for the posv tiers")
+    ap.add_argument("--quick", action="store_true", help="fewer trials / points")
+    a = ap.parse_args(argv)
+    out = pathlib.Path(a.out)
+    out.mkdir(parents=True, exist_ok=True)
+    trials = 20 if a.quick else a.trials
+    kappas = [1e1, 1e2, 1e4, 1e6, 1e8, 1e10, 1e12] if a.quick else [10.0 ** e for e in range(1, 13)]
+    sizes = [2, 8, 32, 64] if a.quick else [2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64]
+    size_kappa = 10.0
+    rng = np.random.default_rng(a.seed)
+    cond_rows, size_rows, tier_rows = [], [], []
+    for prec in PLAIN:
+        for k in kappas:
+            cond_rows += run_plain_point(rng, prec, k, a.n, trials)
+        for n in sizes:
+            size_rows += run_plain_point(rng, prec, size_kappa, n, trials)
+    for k in kappas:
+        for t in TIERS:
+            tier_rows.append(run_tier_point(rng, t, k, a.n_tier, trials))
+    write_csv(out / "error_vs_cond.csv", cond_rows)
+    write_csv(out / "error_vs_n.csv", size_rows)
+    write_csv(out / "tiers_vs_cond.csv", tier_rows)
+    write_report(out / "REPORT.md", cond_rows, size_rows, tier_rows, a.n, a.n_tier, size_kappa, trials, a.seed)
+    print(f"wrote report to {out}/")
+
+
+if __name__ == "__main__":
+    main()
