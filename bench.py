@@ -219,8 +219,15 @@ def run_posv(a):
     # algorithmic bytes per matrix: read A (8n^2), read B and write X (8n*nrhs each), info (4)
     bytes_per = 8 * n * n + 2 * 8 * n * nrhs + 4
     achieved = bytes_per * batch / routed_kern / 1e9
+    traffic = None
+    try:  # HBM bytes per launch measured by rocprofv3 for this workload (profiles/traffic.json)
+        if batch == 131072 and nrhs == 1:
+            traffic = json.loads((ROOT / "profiles" / "traffic.json").read_text())[
+                "posv_n64_b131072_nrhs1"]["k_posv_mx_routed<4>"]
+    except (OSError, KeyError, ValueError):
+        traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_posv_mx_routed<4> (features + network + bf16/fp32 factor on MFMA + float64 refinement "
                           "in one kernel; the event window also holds the fallback-count reset and the gathered "
                           "float64 launch)",
