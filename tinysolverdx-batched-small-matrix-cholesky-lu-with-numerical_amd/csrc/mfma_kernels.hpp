@@ -17,8 +17,13 @@
 //
 // Data kept per wave for the whole solve:
 //   ad   float64 upper blocks of A (80 VGPRs at NBLK = 4): the residual;
-//   w    float factor: V_KK in the diagonal slots, U_KJ elsewhere (40 VGPRs).
-// A is therefore read from HBM once.  Only the lower triangle of A is read
+//   wl   float factor in LDS: V_KK in the diagonal slots, U_KJ elsewhere
+//        (10 KiB per wave at NBLK = 4, each lane owning its 16-byte slot of
+//        every block).
+// A is therefore read from HBM once.  With the factor in LDS the kernels fit
+// 168 registers, i.e. 3 waves per SIMD (amdgpu_waves_per_eu(3)); the few
+// values the compiler then spills are output addresses, reloaded only when X
+// is stored.  Only the lower triangle of A is read
 // (the reference's convention for the Cholesky input): element (r, c) of the
 // symmetric matrix comes from A[max(r,c)][min(r,c)].  Rows and columns past n
 // are identity padding.
@@ -263,40 +268,50 @@ __device__ __forceinline__ double mx_load(const IrArgs& a, int64_t m, int g, int
 // float64 fallback list; info 0 and stat (corrections, or -1) for all.
 template <int NBLK, Low LF>
 __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int j,
-                                         const double (&ad)[nupper<NBLK>()][4], double anorm, double* xs, float* fs) {
+                                         const double (&ad)[nupper<NBLK>()][4], double anorm, double* xs, float* fs,
+                                         v4f* wl) {
   constexpr int NU = nupper<NBLK>();
   const int lane = threadIdx.x;
   const int n = a.n;
-  // ---- factor: float copy, block right-looking, MFMA trailing updates
-  v4f w[NU];
+  // ---- factor: float copy, block right-looking, MFMA trailing updates.
+  // The blocks live in this wave's LDS (NU KiB), each lane owning its 16-byte
+  // slot of every block (only that lane ever reads or writes it, so no
+  // synchronisation); registers hold the float64 blocks, the pivot block's
+  // inverse and one block row at a time, which keeps the kernel at 3 waves
+  // per SIMD at n = 64.
+  auto slot = [&](int b) -> v4f& { return wl[b * kWave + lane]; };
   static_for<0, NU>([&](auto b) {
-    static_for<0, 4>([&](auto v) { w[b][v()] = round_low((float)ad[b][v()], LF); });
+    v4f x;
+    static_for<0, 4>([&](auto v) { x[v()] = round_low((float)ad[b][v()], LF); });
+    slot(b) = x;
   });
   bool bad = false;
   static_for<0, NBLK>([&](auto Kc) {
     constexpr int K = decltype(Kc)::value;
-    const v4f V = diag_inverse<LF>(w[bslot<NBLK>(K, K)], g, j, bad);
-    w[bslot<NBLK>(K, K)] = V;
+    const v4f V = diag_inverse<LF>(slot(bslot<NBLK>(K, K)), g, j, bad);
+    slot(bslot<NBLK>(K, K)) = V;
+    v4f row[NBLK];  // U_KJ, J > K
     static_for<K + 1, NBLK>([&](auto Jc) {
       constexpr int J = decltype(Jc)::value;
+      const v4f wkj = slot(bslot<NBLK>(K, J));
       v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      static_for<0, 4>([&](auto v) {
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(V[v()], w[bslot<NBLK>(K, J)][v()], acc, 0, 0, 0);
-      });
+      static_for<0, 4>([&](auto v) { acc = __builtin_amdgcn_mfma_f32_16x16x4f32(V[v()], wkj[v()], acc, 0, 0, 0); });
       static_for<0, 4>([&](auto v) { acc[v()] = round_low(acc[v()], LF); });
-      w[bslot<NBLK>(K, J)] = acc;
+      row[J] = acc;
+      slot(bslot<NBLK>(K, J)) = acc;
     });
     static_for<K + 1, NBLK>([&](auto Ic) {
       static_for<Ic, NBLK>([&](auto Jc) {
         constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value;
-        v4f acc = w[bslot<NBLK>(I, J)];
+        v4f acc = slot(bslot<NBLK>(I, J));
         static_for<0, 4>([&](auto v) {
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-w[bslot<NBLK>(K, I)][v()], w[bslot<NBLK>(K, J)][v()], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-row[I][v()], row[J][v()], acc, 0, 0, 0);
         });
-        w[bslot<NBLK>(I, J)] = acc;
+        slot(bslot<NBLK>(I, J)) = acc;
       });
     });
   });
+  wave_sync();
 
   // ---- solves with the float factor: t (column form) -> z (column form)
   // one block, column form -> row form, through LDS
@@ -315,12 +330,14 @@ __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int 
       constexpr int K = decltype(Kc)::value;
       float acc = 0.0f;
       static_for<0, K>([&](auto I) {
-        static_for<0, 4>([&](auto v) { acc = __builtin_fmaf(w[bslot<NBLK>(I, K)][v()], yR[I][v()], acc); });
+        const v4f blk = wl[bslot<NBLK>(I, K) * kWave + lane];
+        static_for<0, 4>([&](auto v) { acc = __builtin_fmaf(blk[v()], yR[I][v()], acc); });
       });
       float sR[4];
       c2r(t[K] - sum_g(acc), sR);
       float p = 0.0f;
-      static_for<0, 4>([&](auto v) { p = __builtin_fmaf(w[bslot<NBLK>(K, K)][v()], sR[v()], p); });
+      const v4f dk = wl[bslot<NBLK>(K, K) * kWave + lane];
+      static_for<0, 4>([&](auto v) { p = __builtin_fmaf(dk[v()], sR[v()], p); });
       yc[K] = sum_g(p);
       c2r(yc[K], yR[K]);
     });
@@ -328,11 +345,13 @@ __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int 
       constexpr int K = decltype(Kc)::value;
       float p[4] = {0.0f, 0.0f, 0.0f, 0.0f};
       static_for<K + 1, NBLK>([&](auto J) {
-        static_for<0, 4>([&](auto v) { p[v()] = __builtin_fmaf(w[bslot<NBLK>(K, J)][v()], z[J], p[v()]); });
+        const v4f blk = wl[bslot<NBLK>(K, J) * kWave + lane];
+        static_for<0, 4>([&](auto v) { p[v()] = __builtin_fmaf(blk[v()], z[J], p[v()]); });
       });
       const float wc = yc[K] - bperm(tsum4(p, j), src_mixed);  // column form
       float q[4];
-      static_for<0, 4>([&](auto v) { q[v()] = w[bslot<NBLK>(K, K)][v()] * wc; });
+      const v4f dk = wl[bslot<NBLK>(K, K) * kWave + lane];
+      static_for<0, 4>([&](auto v) { q[v()] = dk[v()] * wc; });
       z[K] = bperm(tsum4(q, j), src_mixed);
     });
   };
@@ -415,15 +434,16 @@ __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int 
 }
 
 template <int NBLK, Low LF>
-__global__ __launch_bounds__(kWave) void k_posv_mx(IrArgs a) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void k_posv_mx(IrArgs a) {
   __shared__ __align__(16) double xs[16 * NBLK];
   __shared__ __align__(16) float fs[16];
+  __shared__ v4f wl[nupper<NBLK>() * kWave];
   const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
   if (a.list && (int)blockIdx.x >= *a.lcount) return;  // uniform: the whole workgroup leaves
   const int64_t m = a.list ? a.list[blockIdx.x] : blockIdx.x;
   double ad[nupper<NBLK>()][4];
   const double anorm = mx_load<NBLK>(a, m, g, j, ad);
-  mx_solve<NBLK, LF>(a, m, g, j, ad, anorm, xs, fs);
+  mx_solve<NBLK, LF>(a, m, g, j, ad, anorm, xs, fs, wl);
 }
 
 // ------------------------------------------------------------ in-wave router
@@ -556,11 +576,12 @@ __device__ __forceinline__ int mx_route(const float* __restrict__ P, const Route
 // tier 2 goes straight to the float64 list (stat -1).  tier_out / feat_out
 // (optional) receive the choice and the features.
 template <int NBLK>
-__global__ __launch_bounds__(kWave) void k_posv_mx_routed(IrArgs a, const float* __restrict__ P,
-                                                          int* __restrict__ tier_out, float* __restrict__ feat_out) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void k_posv_mx_routed(
+    IrArgs a, const float* __restrict__ P, int* __restrict__ tier_out, float* __restrict__ feat_out) {
   __shared__ __align__(16) double xs[16 * NBLK];
   __shared__ __align__(16) float fs[16];
   __shared__ __align__(16) float hs[kRH];
+  __shared__ v4f wl[nupper<NBLK>() * kWave];
   const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
   const int64_t m = blockIdx.x;
   double ad[nupper<NBLK>()][4];
@@ -584,9 +605,9 @@ __global__ __launch_bounds__(kWave) void k_posv_mx_routed(IrArgs a, const float*
     return;
   }
   if (t == 0)
-    mx_solve<NBLK, Low::BF16>(a, m, g, j, ad, anorm, xs, fs);
+    mx_solve<NBLK, Low::BF16>(a, m, g, j, ad, anorm, xs, fs, wl);
   else
-    mx_solve<NBLK, Low::F32>(a, m, g, j, ad, anorm, xs, fs);
+    mx_solve<NBLK, Low::F32>(a, m, g, j, ad, anorm, xs, fs, wl);
 }
 
 }  // namespace tsdx_hip
