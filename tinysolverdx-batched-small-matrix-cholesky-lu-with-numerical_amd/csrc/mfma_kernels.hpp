@@ -217,49 +217,68 @@ __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
 }
 
 // ------------------------------------------------------------------ kernel
-// Loads the upper blocks of matrix m of a.A (see the file comment) and
-// returns ||A||_F of the symmetric matrix.
-template <int NBLK>
-__device__ __forceinline__ double mx_load(const IrArgs& a, int64_t m, int g, int j, double (&ad)[nupper<NBLK>()][4]) {
-  const int n = a.n;
-  const double* Am = a.A + m * (int64_t)n * n;
-  // ---- load the upper blocks of the symmetric matrix (from the lower triangle)
-  // Off-diagonal block (I < J): lane (g, j) holds rows 16I + 4g + v, column
-  // 16J + j, i.e. A[16J + j][16I + 4g .. 4g + 3] (four consecutive entries of
-  // a lower row: two 16-byte loads when rows are 32-byte aligned, vec).
-  // Diagonal blocks read A[max(r,c)][min(r,c)] entry by entry.
-  const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15) == 0);
-  double ss = 0.0;
+// Loads the upper blocks of one matrix (see the file comment) into ad and
+// returns the sum of squares of this lane's entries of the symmetric matrix.
+// Every load is issued unconditionally (entries past n read a valid address,
+// Am[0], and are replaced afterwards), so all of them are in flight at once:
+// a load under a divergent branch is waited for before the branch closes,
+// which would make the wave pay one memory latency per entry.
+// Off-diagonal block (I < J): lane (g, j) holds rows 16I + 4g + v, column
+// 16J + j, i.e. A[16J + j][16I + 4g .. 4g + 3] (four consecutive entries of a
+// lower row: two 16-byte loads when VEC, i.e. rows are 16-byte aligned).
+// Diagonal blocks read A[max(r,c)][min(r,c)] entry by entry.
+template <int NBLK, bool VEC>
+__device__ __forceinline__ double mx_load_blocks(const double* __restrict__ Am, int n, int g, int j,
+                                                 double (&ad)[nupper<NBLK>()][4]) {
   static_for<0, NBLK>([&](auto Ic) {
     static_for<Ic, NBLK>([&](auto Jc) {
       constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
       const int c = 16 * J + j;
-      if (I < J && vec) {
+      if constexpr (I < J && VEC) {
         const int r0 = 16 * I + 4 * g;
-        double2 q0 = {0.0, 0.0}, q1 = {0.0, 0.0};  // past n: identity padding, off the diagonal
-        if (r0 < n && c < n) {
-          const double2* p = reinterpret_cast<const double2*>(Am + (int64_t)c * n + r0);
-          q0 = p[0];
-          q1 = p[1];
-        }
-        ad[b][0] = q0.x; ad[b][1] = q0.y; ad[b][2] = q1.x; ad[b][3] = q1.y;
-        static_for<0, 4>([&](auto v) { ss = __builtin_fma(ad[b][v()], 2.0 * ad[b][v()], ss); });
+        const bool ok = r0 < n && c < n;  // r0 < n implies r0 + 3 < n (n % 4 == 0)
+        const double2* p = reinterpret_cast<const double2*>(Am + (ok ? c * n + r0 : 0));  // < 4096
+        const double2 q0 = p[0], q1 = p[1];
+        ad[b][0] = ok ? q0.x : 0.0;
+        ad[b][1] = ok ? q0.y : 0.0;
+        ad[b][2] = ok ? q1.x : 0.0;
+        ad[b][3] = ok ? q1.y : 0.0;
       } else {
         static_for<0, 4>([&](auto v) {
           const int r = 16 * I + 4 * g + v;
-          double x;
-          if (r < n && c < n) {
-            const int hi = r > c ? r : c, lo = r > c ? c : r;
-            x = Am[(int64_t)hi * n + lo];
-            ss = __builtin_fma(x, (I == J) ? x : 2.0 * x, ss);
-          } else {
-            x = (r == c) ? 1.0 : 0.0;
-          }
-          ad[b][v()] = x;
+          const bool ok = r < n && c < n;
+          const int hi = r > c ? r : c, lo = r > c ? c : r;
+          const double x = Am[ok ? hi * n + lo : 0];
+          ad[b][v()] = ok ? x : ((r == c) ? 1.0 : 0.0);  // past n: identity padding
         });
       }
     });
   });
+  double ss = 0.0;
+  static_for<0, NBLK>([&](auto Ic) {
+    static_for<Ic, NBLK>([&](auto Jc) {
+      constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
+      static_for<0, 4>([&](auto v) {
+        if constexpr (I == J) {  // the padding's unit diagonal is not part of A
+          const double x = (16 * I + 4 * g + v < n) ? ad[b][v()] : 0.0;
+          ss = __builtin_fma(x, x, ss);
+        } else {
+          ss = __builtin_fma(ad[b][v()], 2.0 * ad[b][v()], ss);
+        }
+      });
+    });
+  });
+  return ss;
+}
+
+// Loads the upper blocks of matrix m of a.A and returns ||A||_F of the
+// symmetric matrix.  Only the lower triangle of A is read.
+template <int NBLK>
+__device__ __forceinline__ double mx_load(const IrArgs& a, int64_t m, int g, int j, double (&ad)[nupper<NBLK>()][4]) {
+  const int n = a.n;
+  const double* Am = a.A + m * (int64_t)n * n;
+  const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15) == 0);
+  const double ss = vec ? mx_load_blocks<NBLK, true>(Am, n, g, j, ad) : mx_load_blocks<NBLK, false>(Am, n, g, j, ad);
   return __builtin_sqrt(sum_j(sum_g(ss)));
 }
 
@@ -366,7 +385,8 @@ __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int 
     double bb = 0.0;
     static_for<0, NBLK>([&](auto J) {
       const int row = 16 * J + j;
-      bc[J] = row < n ? Bm[(int64_t)row * a.nrhs + c] : 0.0;
+      const double bv = Bm[row < n ? (int64_t)row * a.nrhs + c : 0];  // unconditional: loads overlap
+      bc[J] = row < n ? bv : 0.0;
       bb = __builtin_fma(bc[J], bc[J], bb);
       t[J] = (float)bc[J];
     });
