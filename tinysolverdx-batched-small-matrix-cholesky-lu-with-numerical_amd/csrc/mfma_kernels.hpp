@@ -545,44 +545,31 @@ __device__ __forceinline__ void mx_features(const double (&ad)[nupper<NBLK>()][4
   f[11] = ddom / (float)n;
 }
 
-// This lane's share of the network weights (lane o = hidden unit o): row o of
-// W1 and of W2, b1[o], b2[o] and column o of W3.
-struct RouterLane {
-  float w1[kRF];
-  float4 w2[kRH / 4];
-  float b1, b2, w3[kRT];
-};
-__device__ __forceinline__ void router_lane_load(const float* __restrict__ P, int lane, RouterLane& r) {
-  static_for<0, kRF>([&](auto q) { r.w1[q] = P[kOffW1 + lane * kRF + q]; });
-  const float4* w2 = reinterpret_cast<const float4*>(P + kOffW2 + lane * kRH);  // kOffW2 is a multiple of 4
-  static_for<0, kRH / 4>([&](auto q) { r.w2[q] = w2[q]; });
-  r.b1 = P[kOffB1 + lane];
-  r.b2 = P[kOffB2 + lane];
-  static_for<0, kRT>([&](auto t) { r.w3[t] = P[kOffW3 + t * kRH + lane]; });
-}
-
-// The network on one wave; returns the tier, the first largest logit (a NaN
-// logit gives tier 2).  hs: 64 floats of LDS.
-__device__ __forceinline__ int mx_route(const float* __restrict__ P, const RouterLane& r, const float (&f)[kRF],
-                                        int lane, float* hs) {
-  float a1 = r.b1;
+// The network on one wave (lane o = hidden unit o); returns the tier, the
+// first largest logit (a NaN logit gives tier 2).  hs: 64 floats of LDS.
+// The weights are read where they are used (row o of W1 and W2, column o of
+// W3), which keeps them out of the registers that hold the matrix.
+__device__ __forceinline__ int mx_route(const float* __restrict__ P, const float (&f)[kRF], int lane, float* hs) {
+  float a1 = P[kOffB1 + lane];
   static_for<0, kRF>([&](auto q) {
-    a1 = __builtin_fmaf(r.w1[q], (f[q] - P[q]) * __builtin_amdgcn_rcpf(P[kRF + q]), a1);
+    a1 = __builtin_fmaf(P[kOffW1 + lane * kRF + q], (f[q] - P[q]) * __builtin_amdgcn_rcpf(P[kRF + q]), a1);
   });
   hs[lane] = a1 > 0.0f ? a1 : 0.0f;
   wave_sync();
-  float a2 = r.b2;
+  float a2 = P[kOffB2 + lane];
+  const float4* w2 = reinterpret_cast<const float4*>(P + kOffW2 + lane * kRH);  // kOffW2 is a multiple of 4
   static_for<0, kRH / 4>([&](auto q) {
+    const float4 w = w2[q];
     const float4 h = *reinterpret_cast<const float4*>(hs + 4 * q);
-    a2 = __builtin_fmaf(r.w2[q].x, h.x, a2);
-    a2 = __builtin_fmaf(r.w2[q].y, h.y, a2);
-    a2 = __builtin_fmaf(r.w2[q].z, h.z, a2);
-    a2 = __builtin_fmaf(r.w2[q].w, h.w, a2);
+    a2 = __builtin_fmaf(w.x, h.x, a2);
+    a2 = __builtin_fmaf(w.y, h.y, a2);
+    a2 = __builtin_fmaf(w.z, h.z, a2);
+    a2 = __builtin_fmaf(w.w, h.w, a2);
   });
   const float h2 = a2 > 0.0f ? a2 : 0.0f;
   float z[kRT];
   static_for<0, kRT>([&](auto t) {
-    z[t] = group_allreduce<64>(r.w3[t] * h2, [](float u, float v) { return u + v; }) + P[kOffB3 + t];
+    z[t] = group_allreduce<64>(P[kOffW3 + t * kRH + lane] * h2, [](float u, float v) { return u + v; }) + P[kOffB3 + t];
   });
   int t = 0;
   t = z[1] > z[0] ? 1 : t;
@@ -613,9 +600,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     static_for<1, kRF>([&](auto q) { v = (lane == q) ? f[q] : v; });
     feat_out[m * kRF + lane] = v;
   }
-  RouterLane rl;
-  router_lane_load(P, lane, rl);
-  const int t = mx_route(P, rl, f, lane, hs);
+  const int t = mx_route(P, f, lane, hs);
   if (tier_out && lane == 0) tier_out[m] = t;
   if (t == 2) {
     if (lane == 0) {
