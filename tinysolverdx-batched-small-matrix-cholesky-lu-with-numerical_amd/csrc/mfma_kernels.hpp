@@ -74,7 +74,8 @@ __device__ __forceinline__ double dppd(double x) {
 }
 // DPP controls (rows of 16 lanes): row_newbcast:K = 0x150 + K (every lane of
 // the row reads lane K of the row), row_ror:8 = 0x128 (partner j ^ 8),
-// row_half_mirror = 0x141 (partner j ^ 7), quad_perm xor 2 = 0x4E, xor 1 = 0xB1.
+// row_half_mirror = 0x141 (partner j ^ 7), row_mirror = 0x140 (partner
+// 15 - j), quad_perm xor 2 = 0x4E, xor 1 = 0xB1.
 template <int K>
 __device__ __forceinline__ float row_bcast(float x) {
   return dppf<0x150 + K>(x);
@@ -112,14 +113,19 @@ __device__ __forceinline__ double sum_g(double x) {
   };
   return swap32(swap16(x));
 }
-// Sum over the 16 lanes of a row group.
-template <class V>
-__device__ __forceinline__ V sum_j(V x) {
-  x += xshfl<1>(x);
-  x += xshfl<2>(x);
-  x += xshfl<7>(x);
-  x += xshfl<15>(x);
-  return x;
+// Sum over the 16 lanes of a row group: partners j^1, j^2, j^7 (row half
+// mirror) and 15-j (row mirror), which together reach every lane of the row.
+__device__ __forceinline__ float sum_j(float x) {
+  x += dppf<0xB1>(x);
+  x += dppf<0x4E>(x);
+  x += dppf<0x141>(x);
+  return x + dppf<0x140>(x);
+}
+__device__ __forceinline__ double sum_j(double x) {
+  x += dppd<0xB1>(x);
+  x += dppd<0x4E>(x);
+  x += dppd<0x141>(x);
+  return x + dppd<0x140>(x);
 }
 
 // Transposed sum over the 16 lanes of a row group: on return lane j holds the
@@ -222,12 +228,13 @@ __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
 // Every load is issued unconditionally (entries past n read a valid address,
 // Am[0], and are replaced afterwards), so all of them are in flight at once:
 // a load under a divergent branch is waited for before the branch closes,
-// which would make the wave pay one memory latency per entry.
+// which would make the wave pay one memory latency per entry.  FULL
+// (n = 16 NBLK) drops the padding selects.
 // Off-diagonal block (I < J): lane (g, j) holds rows 16I + 4g + v, column
 // 16J + j, i.e. A[16J + j][16I + 4g .. 4g + 3] (four consecutive entries of a
 // lower row: two 16-byte loads when VEC, i.e. rows are 16-byte aligned).
 // Diagonal blocks read A[max(r,c)][min(r,c)] entry by entry.
-template <int NBLK, bool VEC>
+template <int NBLK, bool VEC, bool FULL>
 __device__ __forceinline__ double mx_load_blocks(const double* __restrict__ Am, int n, int g, int j,
                                                  double (&ad)[nupper<NBLK>()][4]) {
   static_for<0, NBLK>([&](auto Ic) {
@@ -236,7 +243,7 @@ __device__ __forceinline__ double mx_load_blocks(const double* __restrict__ Am, 
       const int c = 16 * J + j;
       if constexpr (I < J && VEC) {
         const int r0 = 16 * I + 4 * g;
-        const bool ok = r0 < n && c < n;  // r0 < n implies r0 + 3 < n (n % 4 == 0)
+        const bool ok = FULL || (r0 < n && c < n);  // r0 < n implies r0 + 3 < n (n % 4 == 0)
         const double2* p = reinterpret_cast<const double2*>(Am + (ok ? c * n + r0 : 0));  // < 4096
         const double2 q0 = p[0], q1 = p[1];
         ad[b][0] = ok ? q0.x : 0.0;
@@ -246,7 +253,7 @@ __device__ __forceinline__ double mx_load_blocks(const double* __restrict__ Am, 
       } else {
         static_for<0, 4>([&](auto v) {
           const int r = 16 * I + 4 * g + v;
-          const bool ok = r < n && c < n;
+          const bool ok = FULL || (r < n && c < n);
           const int hi = r > c ? r : c, lo = r > c ? c : r;
           const double x = Am[ok ? hi * n + lo : 0];
           ad[b][v()] = ok ? x : ((r == c) ? 1.0 : 0.0);  // past n: identity padding
@@ -254,21 +261,22 @@ __device__ __forceinline__ double mx_load_blocks(const double* __restrict__ Am, 
       }
     });
   });
-  double ss = 0.0;
+  // ||A||_F^2 of this lane's entries: diagonal blocks once, the others twice
+  double sd = 0.0, so = 0.0;
   static_for<0, NBLK>([&](auto Ic) {
     static_for<Ic, NBLK>([&](auto Jc) {
       constexpr int I = decltype(Ic)::value, J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
       static_for<0, 4>([&](auto v) {
         if constexpr (I == J) {  // the padding's unit diagonal is not part of A
-          const double x = (16 * I + 4 * g + v < n) ? ad[b][v()] : 0.0;
-          ss = __builtin_fma(x, x, ss);
+          const double x = (FULL || 16 * I + 4 * g + v < n) ? ad[b][v()] : 0.0;
+          sd = __builtin_fma(x, x, sd);
         } else {
-          ss = __builtin_fma(ad[b][v()], 2.0 * ad[b][v()], ss);
+          so = __builtin_fma(ad[b][v()], ad[b][v()], so);
         }
       });
     });
   });
-  return ss;
+  return __builtin_fma(2.0, so, sd);
 }
 
 // Loads the upper blocks of matrix m of a.A and returns ||A||_F of the
@@ -278,7 +286,13 @@ __device__ __forceinline__ double mx_load(const IrArgs& a, int64_t m, int g, int
   const int n = a.n;
   const double* Am = a.A + m * (int64_t)n * n;
   const bool vec = (n % 4 == 0) && ((reinterpret_cast<uintptr_t>(a.A) & 15) == 0);
-  const double ss = vec ? mx_load_blocks<NBLK, true>(Am, n, g, j, ad) : mx_load_blocks<NBLK, false>(Am, n, g, j, ad);
+  double ss;
+  if (vec && n == 16 * NBLK)
+    ss = mx_load_blocks<NBLK, true, true>(Am, n, g, j, ad);
+  else if (vec)
+    ss = mx_load_blocks<NBLK, true, false>(Am, n, g, j, ad);
+  else
+    ss = mx_load_blocks<NBLK, false, false>(Am, n, g, j, ad);
   return __builtin_sqrt(sum_j(sum_g(ss)));
 }
 
@@ -469,12 +483,20 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
 // ------------------------------------------------------------ in-wave router
 // The 12 router features (definitions in router_kernels.hpp, k_router_features)
 // of the symmetric matrix held in ad; every lane gets every feature.
+// Off-diagonal |a| sums are accumulated per lane in float64 as the blocks are
+// held (the diagonal left out, so r_i needs no cancellation), rounded to float
+// and transposed through LDS so that lane i ends up with row i's r_i and
+// a_ii; the per-row work and the reductions then run once over 64 lanes.
+//   tl: 256 NBLK floats of LDS, ul: 80 NBLK floats of LDS.
 template <int NBLK>
 __device__ __forceinline__ void mx_features(const double (&ad)[nupper<NBLK>()][4], int n, int g, int j, double anorm,
-                                            float (&f)[kRF]) {
-  // (sums in float64 as the blocks are held; float from the per-row values on)
-  const int src_mixed = 16 * (j >> 2) + 4 * (j & 3);
-  double rabs[NBLK], dg[NBLK], pc[NBLK];
+                                            float (&f)[kRF], float* tl, float* ul) {
+  constexpr int NR = 16 * NBLK;  // rows of the padded matrix
+  const int lane = 16 * g + j;
+  // p[v]: row 16I + 4g + v over this lane's column of the blocks (I, J >= I);
+  // pc[J]: column 16J + j over rows 4g + v of the blocks (I < J), i.e. by
+  // symmetry row 16J + j over the blocks left of the diagonal.
+  double pc[NBLK];
   static_for<0, NBLK>([&](auto J) { pc[J] = 0.0; });
   static_for<0, NBLK>([&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
@@ -482,67 +504,81 @@ __device__ __forceinline__ void mx_features(const double (&ad)[nupper<NBLK>()][4
     static_for<I, NBLK>([&](auto Jc) {
       constexpr int J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
       static_for<0, 4>([&](auto v) {
-        p[v] += __builtin_fabs(ad[b][v()]);
-        if constexpr (I < J) pc[J] += __builtin_fabs(ad[b][v()]);
+        const double x = __builtin_fabs(ad[b][v()]);
+        if constexpr (I == J) {
+          p[v] += (4 * g + v == j) ? 0.0 : x;
+        } else {
+          p[v] += x;
+          pc[J] += x;
+        }
       });
     });
-    rabs[I] = bperm(tsum4(p, j), src_mixed);  // row sums of |a| through the upper blocks, column form
-    double dv = 0.0;
-    static_for<0, 4>([&](auto v) { dv = (4 * g + v == j) ? ad[bslot<NBLK>(I, I)][v()] : dv; });
-    dg[I] = sum_g(dv);  // diagonal, column form
+    // tl[j][row]: the partial sums of lane column j
+    const float4 q = {(float)p[0], (float)p[1], (float)p[2], (float)p[3]};
+    *reinterpret_cast<float4*>(tl + j * NR + 16 * I + 4 * g) = q;
+    static_for<0, 4>([&](auto v) {
+      if (4 * g + v == j) ul[4 * NR + 16 * I + j] = (float)ad[bslot<NBLK>(I, I)][v()];
+    });
   });
+  static_for<0, NBLK>([&](auto J) { ul[g * NR + 16 * J + j] = (float)pc[J]; });
+  wave_sync();
+  // lane = row index
+  const bool act = lane < n;  // n <= NR
+  const int e = act ? lane : 0;
+  float r = 0.0f;
+  static_for<0, 16>([&](auto jj) { r += tl[jj * NR + e]; });
+  static_for<0, 4>([&](auto gg) { r += ul[gg * NR + e]; });
+  const float d = ul[4 * NR + e];
+  wave_sync();
   const float big = 3.0e38f;
-  float dmin = big, dmax = 0.0f, ninf = 0.0f, gers = big, dom = 0.0f, dsum = 0.0f, lsum = 0.0f, nonpos = 0.0f,
-        ddom = 0.0f;
-  static_for<0, NBLK>([&](auto J) {
-    const bool act = 16 * J + j < n;
-    const double ra = rabs[J] + sum_g(pc[J]);
-    const float d = (float)dg[J], r = (float)(ra - __builtin_fabs(dg[J]));
-    if (act) {
-      dmin = d < dmin ? d : dmin;
-      dmax = __builtin_fabsf(d) > dmax ? __builtin_fabsf(d) : dmax;
-      ninf = (float)ra > ninf ? (float)ra : ninf;
-      gers = d - r < gers ? d - r : gers;
-      const float q = d > 0.0f ? r * __builtin_amdgcn_rcpf(d) : big;
-      dom = q > dom ? q : dom;
-      dsum += d;
-      lsum += flog10(d);
-      nonpos = (d > 0.0f) ? nonpos : 1.0f;
-      ddom += (d > r) ? 1.0f : 0.0f;
-    }
-  });
-  auto red = [](float x, auto op) {
-    x = op(x, xshfl<1>(x));
-    x = op(x, xshfl<2>(x));
-    x = op(x, xshfl<7>(x));
-    return op(x, xshfl<15>(x));
+  const float ra = r + __builtin_fabsf(d);
+  float dmin = act ? d : big;
+  float dmax = act ? __builtin_fabsf(d) : 0.0f;
+  float ninf = act ? ra : 0.0f;
+  float gers = act ? d - r : big;
+  float dom = act ? (d > 0.0f ? r * __builtin_amdgcn_rcpf(d) : big) : 0.0f;
+  float dsum = act ? d : 0.0f;
+  float lsum = act ? flog10(d) : 0.0f;
+  float nonpos = (act && !(d > 0.0f)) ? 1.0f : 0.0f;
+  float ddom = (act && d > r) ? 1.0f : 0.0f;
+  // reductions over the 64 lanes, the nine in lockstep (independent
+  // instructions between each lane move and its use): rows of 16 by DPP,
+  // then across rows by permlane swaps.  Kinds: 0 min, 1 max, 2 sum.
+  float v[9] = {dmin, dmax, ninf, gers, dom, dsum, lsum, nonpos, ddom};
+  constexpr int kind[9] = {0, 1, 1, 0, 1, 2, 2, 1, 2};
+  auto op = [&](int q, float x, float y) {
+    return kind[q] == 0 ? __builtin_fminf(x, y) : kind[q] == 1 ? __builtin_fmaxf(x, y) : x + y;
   };
-  auto fsum = [](float x, float y) { return x + y; };
-  auto fmin = [](float x, float y) { return x < y ? x : y; };
-  auto fmax = [](float x, float y) { return x > y ? x : y; };
-  dmin = red(dmin, fmin);
-  dmax = red(dmax, fmax);
-  ninf = red(ninf, fmax);
-  gers = red(gers, fmin);
-  dom = red(dom, fmax);
-  dsum = red(dsum, fsum);
-  lsum = red(lsum, fsum);
-  nonpos = red(nonpos, fmax);
-  ddom = red(ddom, fsum);
-  const float fr = (float)anorm;
-  f[0] = (float)n / 64.0f;
+  static_for<0, 9>([&](auto q) { v[q] = op(q, v[q], dppf<0xB1>(v[q])); });
+  static_for<0, 9>([&](auto q) { v[q] = op(q, v[q], dppf<0x4E>(v[q])); });
+  static_for<0, 9>([&](auto q) { v[q] = op(q, v[q], dppf<0x141>(v[q])); });
+  static_for<0, 9>([&](auto q) { v[q] = op(q, v[q], dppf<0x140>(v[q])); });
+  static_for<0, 9>([&](auto q) {
+    const unsigned u = __builtin_bit_cast(unsigned, v[q]);
+    const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    v[q] = op(q, __builtin_bit_cast(float, (unsigned)a[0]), __builtin_bit_cast(float, (unsigned)a[1]));
+  });
+  static_for<0, 9>([&](auto q) {
+    const unsigned u = __builtin_bit_cast(unsigned, v[q]);
+    const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    v[q] = op(q, __builtin_bit_cast(float, (unsigned)a[0]), __builtin_bit_cast(float, (unsigned)a[1]));
+  });
+  dmin = v[0], dmax = v[1], ninf = v[2], gers = v[3], dom = v[4], dsum = v[5], lsum = v[6], nonpos = v[7],
+  ddom = v[8];
+  const float fr = (float)anorm, rn = __builtin_amdgcn_rcpf((float)n);
+  f[0] = (float)n * (1.0f / 64.0f);
   f[1] = flog10(dmin);
   f[2] = flog10(dmax);
   f[3] = flog10(fr);
   f[4] = flog10(ninf);
-  f[5] = fr > 0.0f ? gers / fr : 0.0f;
+  f[5] = fr > 0.0f ? gers * __builtin_amdgcn_rcpf(fr) : 0.0f;
   f[5] = f[5] < -30.0f ? -30.0f : f[5];
   f[6] = dom >= big ? 30.0f : flog10(dom);
-  f[7] = dmin > 0.0f ? flog10(dmax / dmin) : 30.0f;
-  f[8] = dsum > 0.0f ? flog10(fr / (dsum / (float)n)) : 30.0f;
-  f[9] = lsum / (float)n - flog10(dmax);
+  f[7] = dmin > 0.0f ? flog10(dmax * __builtin_amdgcn_rcpf(dmin)) : 30.0f;
+  f[8] = dsum > 0.0f ? flog10(fr * (float)n * __builtin_amdgcn_rcpf(dsum)) : 30.0f;
+  f[9] = lsum * rn - f[2];
   f[10] = nonpos;
-  f[11] = ddom / (float)n;
+  f[11] = ddom * rn;
 }
 
 // The network on one wave (lane o = hidden unit o); returns the tier, the
@@ -589,12 +625,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
   __shared__ __align__(16) float fs[16];
   __shared__ __align__(16) float hs[kRH];
   __shared__ v4f wl[nupper<NBLK>() * kWave];
+  __shared__ __align__(16) float ul[80 * NBLK];
   const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
   const int64_t m = blockIdx.x;
   double ad[nupper<NBLK>()][4];
   const double anorm = mx_load<NBLK>(a, m, g, j, ad);
   float f[kRF];
-  mx_features<NBLK>(ad, a.n, g, j, anorm, f);
+  mx_features<NBLK>(ad, a.n, g, j, anorm, f, reinterpret_cast<float*>(wl), ul);
   if (feat_out && lane < kRF) {
     float v = f[0];
     static_for<1, kRF>([&](auto q) { v = (lane == q) ? f[q] : v; });

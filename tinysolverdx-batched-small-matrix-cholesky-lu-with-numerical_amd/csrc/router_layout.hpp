@@ -13,7 +13,11 @@ constexpr int kOffW1 = 24, kOffB1 = kOffW1 + kRH * kRF, kOffW2 = kOffB1 + kRH, k
               kOffW3 = kOffB2 + kRH, kOffB3 = kOffW3 + kRT * kRH, kRouterParams = kOffB3 + kRT;
 static_assert(kRouterParams == 5211, "parameter layout");
 
-// log10 with non-positive (and tiny) arguments clamped to -30
-__device__ __forceinline__ float flog10(float x) { return x > 1e-30f ? __log10f(x) : -30.0f; }
+// log10 with non-positive (and tiny) arguments clamped to -30: log2 on the
+// transcendental unit (arguments past the clamp are normal numbers, which it
+// handles to about 1 ulp) times log10(2).
+__device__ __forceinline__ float flog10(float x) {
+  return x > 1e-30f ? __builtin_amdgcn_logf(x) * 0.30102999566398120f : -30.0f;
+}
 
 }  // namespace tsdx_hip
