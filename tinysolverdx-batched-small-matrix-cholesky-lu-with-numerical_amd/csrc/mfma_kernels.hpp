@@ -167,6 +167,49 @@ __device__ __forceinline__ V tsum4(const V (&p)[4], int j) {
   return c;
 }
 
+// x[v] += bcast_K(x[v]) * s, v = 0..3, where bcast_K reads lane K of the
+// lane's row of 16 (DPP row_newbcast:K): four v_fmac_f32 with the lane move as
+// their DPP source operand.  The compiler folds DPP moves into adds and
+// multiplies but not into FMAs, where the move would cost a second VALU
+// instruction; the s_nop 1 covers the two wait states a DPP read needs after a
+// VALU write of its source.  Same arithmetic as fmaf(bcast_K(x), s, x).
+template <int K>
+__device__ __forceinline__ void bcast_fmac4(v4f& x, float s);
+#define TSDX_BCAST_FMAC4(K)                                                                        \
+  template <>                                                                                      \
+  __device__ __forceinline__ void bcast_fmac4<K>(v4f & x, float s) {                              \
+    float x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];                                              \
+    asm volatile(                                                                                  \
+        "s_nop 1\n\t"                                                                              \
+        "v_fmac_f32_dpp %0, %0, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t" \
+        "v_fmac_f32_dpp %1, %1, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t" \
+        "v_fmac_f32_dpp %2, %2, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t" \
+        "v_fmac_f32_dpp %3, %3, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf bound_ctrl:1"     \
+        : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3)                                                   \
+        : "v"(s));                                                                                 \
+    x[0] = x0;                                                                                     \
+    x[1] = x1;                                                                                     \
+    x[2] = x2;                                                                                     \
+    x[3] = x3;                                                                                     \
+  }
+TSDX_BCAST_FMAC4(0)
+TSDX_BCAST_FMAC4(1)
+TSDX_BCAST_FMAC4(2)
+TSDX_BCAST_FMAC4(3)
+TSDX_BCAST_FMAC4(4)
+TSDX_BCAST_FMAC4(5)
+TSDX_BCAST_FMAC4(6)
+TSDX_BCAST_FMAC4(7)
+TSDX_BCAST_FMAC4(8)
+TSDX_BCAST_FMAC4(9)
+TSDX_BCAST_FMAC4(10)
+TSDX_BCAST_FMAC4(11)
+TSDX_BCAST_FMAC4(12)
+TSDX_BCAST_FMAC4(13)
+TSDX_BCAST_FMAC4(14)
+TSDX_BCAST_FMAC4(15)
+#undef TSDX_BCAST_FMAC4
+
 // ------------------------------------------------- diagonal block elimination
 // W: symmetric 16 x 16 block (accumulator layout).  Returns V = U^-1 where
 // W = U^T U (U upper, entries rounded to the low format), and sets bad when a
@@ -205,7 +248,7 @@ __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
     float t;                                   // T[k][j]
     if constexpr (LF == Low::F32) {
       t = wk * (rd * rd);
-      static_for<0, 4>([&](auto v) { W[v()] = __builtin_fmaf(-row_bcast<k>(W[v()]), t, W[v()]); });
+      bcast_fmac4<k>(W, -t);
     } else {  // U entries rounded to the low format
       const float uj = round_low(wk * rd, LF);
       t = uj * rd;
@@ -216,7 +259,7 @@ __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
     }
     const float tm = (j > k) ? t : 0.0f;
     rdl = (j == k) ? rd : rdl;
-    static_for<0, 4>([&](auto v) { F[v()] = __builtin_fmaf(-tm, row_bcast<k>(F[v()]), F[v()]); });
+    bcast_fmac4<k>(F, -tm);
   });
   static_for<0, 4>([&](auto v) { F[v()] *= rdl; });
   return F;
