@@ -120,3 +120,48 @@ def test_padded_sizes(tsdx, n):
     X, info, stat = _run(A, B, "fp32")
     assert (info == 0).all() and (stat >= 0).mean() > 0.9, np.bincount(stat + 1)
     assert _bwd(tsdx, A, X, B).max() <= 1e-14
+
+
+@pytest.mark.parametrize("n", [64, 40])
+def test_fp64_block_tier_reports_info_like_plain_posv(tsdx, oracle, n):
+    """For n > 32 the float64 tier is a block Cholesky on the matrix cores; a
+    matrix it cannot factor (not SPD) is finished by the plain float64 posv in
+    the same wave, so info is the reference's."""
+    rng = np.random.default_rng(40 + n)
+    A, B, _ = _system(rng, 16, n, 1, 3, 1)
+    A[3] = -np.eye(n)
+    A[7, n - 2, n - 2] = -3.0
+    A[11, 20, 20] = 0.0
+    A[11, 20, :20] = 0.0
+    X, info, stat = _run(A, B, "fp32")
+    _, _, info_o = oracle.posv(A, B)
+    assert np.array_equal(info, info_o) and info[3] == 1 and info[7] > 0 and info[11] > 0
+    assert stat[3] == -1 and stat[7] == -1 and stat[11] == -1
+    ok = info == 0
+    assert _bwd(tsdx, A[ok], X[ok], B[ok]).max() <= 1e-14
+
+
+@pytest.mark.parametrize("n,nrhs", [(64, 3), (50, 1), (37, 5)])
+def test_fp64_block_tier_meets_tolerance(tsdx, n, nrhs):
+    """bf16 factors cannot be refined at kappa 1e5..1e10: most matrices go to
+    the float64 tier, whose answers must still meet the bar for every
+    right-hand side (kappa up to 1e10 also exercises its own refinement)."""
+    rng = np.random.default_rng(70 + n)
+    A, B, kappa = _system(rng, 96, n, 5, 10, nrhs)
+    X, info, stat = _run(A, B, "bf16")
+    assert (info == 0).all() and (stat == -1).sum() > 48, np.bincount(stat + 1)
+    bw = _bwd(tsdx, A, X, B)
+    assert bw.max() <= 1e-14, (bw.max(), kappa[bw.argmax()])
+
+
+def test_fp64_block_tier_reads_lower_triangle_only(tsdx):
+    rng = np.random.default_rng(8)
+    A, B, _ = _system(rng, 32, 64, 6, 10, 2)
+    G = A.copy()
+    iu = np.triu_indices(64, 1)
+    G[:, iu[0], iu[1]] = 1e30
+    X, info, stat = _run(G, B, "bf16")
+    Xr, info_r, stat_r = _run(A, B, "bf16")
+    assert (stat == -1).sum() > 16
+    assert np.array_equal(X, Xr) and np.array_equal(stat, stat_r) and (info == 0).all()
+    assert _bwd(tsdx, A, X, B).max() <= 1e-14

@@ -213,7 +213,11 @@ TSDX_BCAST_FMAC4(15)
 // ------------------------------------------------- diagonal block elimination
 // W: symmetric 16 x 16 block (accumulator layout).  Returns V = U^-1 where
 // W = U^T U (U upper, entries rounded to the low format), and sets bad when a
-// pivot is not positive (the pivot is then taken as 1).
+// pivot is not positive.  The pivots are not tested step by step: a pivot
+// that is not positive (or NaN) makes its 1/sqrt NaN or infinite, and every
+// lane holds 1/U[j][j] of its column at the end, so one test of those 16
+// values after the last step finds it (a NaN may then spread through the
+// block, which is dropped with the matrix).
 //
 // With U = D T (D = diag(U), T unit upper, T[k][j] = W_k[k][j] / W_k[k][k]
 // for the partly reduced W_k), step k does
@@ -241,26 +245,25 @@ __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
     // an element of a vector type reads element 0)
     const float wkv = W[kv];
     const float akk = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wkv), 16 * kg + k));
-    const bool nb = !(akk > 0.0f);
-    bad = bad || nb;
-    const float rd = __builtin_amdgcn_rsqf(nb ? 1.0f : akk);
+    const float rd = __builtin_amdgcn_rsqf(akk);
     const float wk = bperm(wkv, 16 * kg + j);  // W[k][j], from row k (lane 16 kg + j)
-    float t;                                   // T[k][j]
+    float nt;                                  // -T[k][j]
     if constexpr (LF == Low::F32) {
-      t = wk * (rd * rd);
-      bcast_fmac4<k>(W, -t);
+      nt = wk * (-rd * rd);
+      bcast_fmac4<k>(W, nt);
     } else {  // U entries rounded to the low format
       const float uj = round_low(wk * rd, LF);
-      t = uj * rd;
+      nt = uj * -rd;
       static_for<0, 4>([&](auto v) {
         const float ui = round_low(row_bcast<k>(W[v()]) * rd, LF);
         W[v()] = __builtin_fmaf(-ui, uj, W[v()]);
       });
     }
-    const float tm = (j > k) ? t : 0.0f;
     rdl = (j == k) ? rd : rdl;
-    bcast_fmac4<k>(F, -tm);
+    bcast_fmac4<k>(F, (j > k) ? nt : 0.0f);
   });
+  // every pivot positive and finite <=> every 1/sqrt positive and finite
+  bad = bad || __builtin_amdgcn_ballot_w64(!(rdl > 0.0f && rdl < __builtin_huge_valf())) != 0;
   static_for<0, 4>([&](auto v) { F[v()] *= rdl; });
   return F;
 }
@@ -447,13 +450,23 @@ __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int 
       bb = __builtin_fma(bc[J], bc[J], bb);
       t[J] = (float)bc[J];
     });
-    const double bnorm = __builtin_sqrt(sum_j(bb));
+    const double bb2 = sum_j(bb);  // ||b||^2
     solve(t, z);
     static_for<0, NBLK>([&](auto J) { xc[J] = (16 * J + j < n) ? (double)z[J] : 0.0; });
     bool conv = false;
     double prev = __builtin_huge_val();
+    // The test ||r|| <= tol/2 (||A||_F ||x|| + ||b||) on squares, with no
+    // square root: where it can pass, r is tiny and ||A||_F ||x|| >= ||A x||
+    // >= ||b|| - ||r||, so (||A|| ||x|| + ||b||)^2 >= ||A||^2 ||x||^2 +
+    // 3 ||b||^2 (to rounding), and ||r||^2 <= tol^2/4 (||A||^2 ||x||^2 +
+    // 3 ||b||^2) implies the test.
+    const double ctol = 0.25 * a.tol * a.tol, aa2 = anorm * anorm, bb3 = 3.0 * bb2;
     for (int it = 0; it <= a.max_iter; ++it) {
-      // r = b - A x in float64 with the stored upper blocks
+      // r = b - A x in float64 with the stored upper blocks.  A diagonal block
+      // is symmetric, so its product is taken as column sums (a sum over the
+      // lane's four rows and then over the four row groups); an off-diagonal
+      // block serves both as A_IJ (row sums over the 16 lanes of a row group)
+      // and as A_JI = A_IJ^T (column sums).
       if (g == 0) static_for<0, NBLK>([&](auto J) { xs[16 * J + j] = xc[J]; });
       wave_sync();
       double pc[NBLK], rc[NBLK];
@@ -463,15 +476,20 @@ __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int 
         const double2 q0 = *reinterpret_cast<const double2*>(xs + 16 * I + 4 * g);
         const double2 q1 = *reinterpret_cast<const double2*>(xs + 16 * I + 4 * g + 2);
         const double xR[4] = {q0.x, q0.y, q1.x, q1.y};  // row form of block I
-        double p[4] = {0.0, 0.0, 0.0, 0.0};
-        static_for<I, NBLK>([&](auto Jc) {
-          constexpr int J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
-          static_for<0, 4>([&](auto v) {
-            p[v] = __builtin_fma(ad[b][v()], xc[J], p[v]);
-            if constexpr (I < J) pc[J] = __builtin_fma(ad[b][v()], xR[v], pc[J]);
+        static_for<0, 4>([&](auto v) { pc[I] = __builtin_fma(ad[bslot<NBLK>(I, I)][v()], xR[v], pc[I]); });
+        if constexpr (I + 1 < NBLK) {
+          double p[4] = {0.0, 0.0, 0.0, 0.0};
+          static_for<I + 1, NBLK>([&](auto Jc) {
+            constexpr int J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
+            static_for<0, 4>([&](auto v) {
+              p[v] = __builtin_fma(ad[b][v()], xc[J], p[v]);
+              pc[J] = __builtin_fma(ad[b][v()], xR[v], pc[J]);
+            });
           });
-        });
-        rc[I] = bperm(tsum4(p, j), src_mixed);  // rows of block I through upper blocks, column form
+          rc[I] = bperm(tsum4(p, j), src_mixed);  // rows of block I right of the diagonal, column form
+        } else {
+          rc[I] = 0.0;
+        }
       });
       wave_sync();
       double rr = 0.0, xx = 0.0;
@@ -480,14 +498,14 @@ __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int 
         rr = __builtin_fma(rc[J], rc[J], rr);
         xx = __builtin_fma(xc[J], xc[J], xx);
       });
-      const double rn = __builtin_sqrt(sum_j(rr)), xn = __builtin_sqrt(sum_j(xx));
-      if (rn <= 0.5 * a.tol * (anorm * xn + bnorm)) {
+      const double rn2 = sum_j(rr), xn2 = sum_j(xx);
+      if (rn2 <= ctol * __builtin_fma(aa2, xn2, bb3)) {
         conv = true;
         used = used > it ? used : it;
         break;
       }
-      if (it == a.max_iter || !(rn < 0.5 * prev)) break;
-      prev = rn;
+      if (it == a.max_iter || !(rn2 < 0.25 * prev)) break;
+      prev = rn2;
       static_for<0, NBLK>([&](auto J) { t[J] = (float)rc[J]; });
       solve(t, z);
       static_for<0, NBLK>([&](auto J) { xc[J] += (16 * J + j < n) ? (double)z[J] : 0.0; });

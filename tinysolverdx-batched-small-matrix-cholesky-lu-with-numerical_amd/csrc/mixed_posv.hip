@@ -1,7 +1,7 @@
 // Launchers of the mixed-precision refinement tier (mixed_kernels.hpp).
 #include "capi_internal.hpp"
 #include "launch_impl.hpp"
-#include "mfma_kernels.hpp"
+#include "mfma64_kernels.hpp"
 
 namespace tsdx_hip {
 
@@ -31,9 +31,24 @@ static void launch_mx(const IrArgs& a, int low, hipStream_t s) {
     hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F32>), grid, block, 0, s, a);
 }
 
-// float64 posv of the matrices on the fallback list (grid-stride over it)
+// float64 posv of the matrices on the fallback list (grid-stride over it):
+// for n > 32 the float64 block kernel on the matrix cores (k_posv_mx64, which
+// hands its own failures to the plain posv in the same wave) unless `plain`
+// asks for the plain posv throughout; the plain posv for smaller sizes, whose
+// buckets put several matrices in a wave.
 template <int NB>
-static hipError_t gather(const IrArgs& a, hipStream_t s) {
+static hipError_t gather(const IrArgs& a, hipStream_t s, bool plain) {
+  if constexpr (NB == 64) {
+    if (!plain) {
+      // two waves per SIMD fit this kernel: 4096 workgroups fill the GPU twice
+      const dim3 gg((unsigned)(a.batch < 4096 ? a.batch : 4096));
+      if (a.nrhs <= 1)
+        hipLaunchKernelGGL((k_posv_mx64<4, 1>), gg, dim3(kWave), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_posv_mx64<4, 4>), gg, dim3(kWave), 0, s, a);
+      return hipGetLastError();
+    }
+  }
   const dim3 g = grid_for<NB>(a.batch);
   const dim3 gg(g.x < 16384u ? g.x : 16384u);
   if (a.nrhs <= 1)
@@ -44,11 +59,11 @@ static hipError_t gather(const IrArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_posv_gather(const IrArgs& a, hipStream_t s) {
-  if (a.n <= 4) return gather<4>(a, s);
-  if (a.n <= 8) return gather<8>(a, s);
-  if (a.n <= 16) return gather<16>(a, s);
-  if (a.n <= 32) return gather<32>(a, s);
-  return gather<64>(a, s);
+  if (a.n <= 4) return gather<4>(a, s, false);
+  if (a.n <= 8) return gather<8>(a, s, false);
+  if (a.n <= 16) return gather<16>(a, s, false);
+  if (a.n <= 32) return gather<32>(a, s, false);
+  return gather<64>(a, s, false);
 }
 
 template <int NB>
@@ -59,7 +74,7 @@ static hipError_t launch_ir(const IrArgs& a, int low, hipStream_t s) {
   } else {  // float64 direct for every matrix: the list is 0..batch-1
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((a.batch + 255) / 256)), dim3(256), 0, s, a.flist, a.fcount, a.stat, a.batch);
   }
-  return gather<NB>(a, s);
+  return gather<NB>(a, s, low == 4);
 }
 
 hipError_t launch_posv_ir(const IrArgs& a, int low, hipStream_t s) {
