@@ -512,8 +512,11 @@ __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int 
     }
     ok = conv;
     if (conv && g == 0) {
+      // the store offsets are formed here (opaque lane index): shared with
+      // the loads of B they would stay live through the refinement and spill
+      const int jj = opaque(j);
       static_for<0, NBLK>([&](auto J) {
-        const int row = 16 * J + j;
+        const int row = 16 * J + jj;
         if (row < n) Xm[(int64_t)row * a.nrhs + c] = xc[J];
       });
     }
