@@ -66,8 +66,16 @@ struct ColumnDivider<double, R> {
 // Factor in registers.  r[s][j] = A(q + L*s, j) on entry, L on exit (entries
 // above the diagonal are garbage).  col: per-wave LDS, 2 * 64 * R elements.
 // nloop / n as for chol_regs (plain_kernels.hpp).
-template <class T, int L, int R>
-__device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int nloop, int n, int q, int g, T* col) {
+// done(ic<s>) is called as soon as slot s < R - 1 holds final rows (after the
+// pivot of row L*s + L - 1), so that a caller can store them while the
+// factorization goes on; later steps leave those rows' lower parts alone.
+struct NoSlotHook {
+  template <class S>
+  __device__ __forceinline__ void operator()(S) const {}
+};
+template <class T, int L, int R, class Done = NoSlotHook>
+__device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int nloop, int n, int q, int g, T* col,
+                                         Done done = Done{}) {
   constexpr int N = L * R;
   int info = 0;
   auto pivot = [&](auto kc) {
@@ -102,6 +110,10 @@ __device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int nloop, int n, int
         static_for<SA, R>([&](auto s) { r[s][j] = fmaT(-r[s][k], c, r[s][j]); });
       });
     }
+    // rows L*s .. L*s + L - 1 are final once pivot(L*s + L - 1) has run (in
+    // step k = L*s + L - 2, or never when nloop stops earlier, in which case
+    // those rows are at or beyond nloop and only padding)
+    if constexpr ((k + 2) % L == 0 && (k + 2) / L - 1 < R - 1) done(ic<(k + 2) / L - 1>{});
   });
   return info;
 }
@@ -145,11 +157,15 @@ __global__ __launch_bounds__(kWave * WPB) void k_potrf_rows(T* __restrict__ A, i
     const int row = q + L * s;
     load_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, live && row < n, vec != 0);
   });
-  const int inf = chol_rows<T, L, R>(r, n, n, q, g, col_s[wv]);
-  static_for<0, R>([&](auto s) {
+  // each slot is stored as soon as its rows are final: the writes of the
+  // first R - 1 slots overlap the rest of the factorization
+  auto store_slot = [&](auto sc) {
+    constexpr int s = decltype(sc)::value;
     const int row = q + L * s;
     if (live && row < n) store_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, vec != 0, true, row);
-  });
+  };
+  const int inf = chol_rows<T, L, R>(r, n, n, q, g, col_s[wv], store_slot);
+  store_slot(ic<R - 1>{});
   if (live && q == 0 && info) info[m] = inf;
 }
 
