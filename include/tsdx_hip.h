@@ -102,11 +102,14 @@ int tsdx_hip_gesv_solve_f64(const double *A, int n, int64_t batch, const double 
  * `factor`, each right-hand side refined with float64 residuals until the
  * normwise backward error ||b - Ax|| / (||A||_F ||x|| + ||b||) <= tol, at most
  * max_iter corrections; matrices whose low-precision factorization fails or
- * whose refinement does not reach tol are solved again by the plain float64
- * posv (the fallback), so every X meets tol unless A itself is not SPD in
- * float64 (then info > 0, as for tsdx_hip_posv_f64).
+ * whose refinement does not reach tol are solved again in float64 (the
+ * fallback: for n > 32 a block Cholesky on the matrix cores with float64
+ * refinement, itself backed by the plain float64 posv in the same wave; the
+ * plain float64 posv for n <= 32), so every X meets tol unless A itself is
+ * not SPD in float64 (then info > 0, as for tsdx_hip_posv_f64).
  *   factor: TSDX_HIP_FACTOR_FP32 / _BF16 / _FP16 (low precision + refinement),
- *           _FP64 (plain float64 solve for every matrix), _AUTO (FP32).
+ *           _FP64 (plain float64 solve for every matrix), _FP64_BLOCK (the
+ *           float64 fallback tier for every matrix), _AUTO (FP32).
  *   stat:   (batch) or NULL: corrections used, or -1 for a matrix that took
  *           the float64 fallback.
  *   work:   device workspace of tsdx_hip_posv_ir_workspace_size(batch) bytes
@@ -117,6 +120,7 @@ int tsdx_hip_gesv_solve_f64(const double *A, int n, int64_t batch, const double 
 #define TSDX_HIP_FACTOR_BF16 2
 #define TSDX_HIP_FACTOR_FP16 3
 #define TSDX_HIP_FACTOR_FP64 4
+#define TSDX_HIP_FACTOR_FP64_BLOCK 5
 
 size_t tsdx_hip_posv_ir_workspace_size(int64_t batch);
 int tsdx_hip_posv_ir_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs, int factor,

@@ -38,7 +38,7 @@ def _bwd(tsdx, A, X, B):
     return np.stack([tsdx.solve_backward_error(A, X[..., r], B[..., r]) for r in range(B.shape[-1])]).max(axis=0)
 
 
-@pytest.mark.parametrize("factor", ["fp32", "bf16", "fp16", "fp64"])
+@pytest.mark.parametrize("factor", ["fp32", "bf16", "fp16", "fp64", "fp64_block"])
 @pytest.mark.parametrize("n,nrhs", [(64, 1), (64, 4), (32, 2), (16, 8), (5, 1)])
 def test_every_answer_meets_tolerance(tsdx, factor, n, nrhs):
     rng = np.random.default_rng(n * 10 + nrhs)
@@ -47,7 +47,7 @@ def test_every_answer_meets_tolerance(tsdx, factor, n, nrhs):
     assert (info == 0).all()
     bw = _bwd(tsdx, A, X, B)
     assert bw.max() <= 1e-14, (factor, n, bw.max(), stat[bw.argmax()], kappa[bw.argmax()])
-    if factor == "fp64":
+    if factor in ("fp64", "fp64_block"):
         assert (stat == -1).all()
     else:
         assert ((stat >= 0) & (stat <= 10) | (stat == -1)).all()

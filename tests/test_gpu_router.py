@@ -198,3 +198,29 @@ def test_fused_features_match_standalone_kernel():
         got = feat.cpu().numpy()
         assert np.allclose(got, router.features(dA).cpu().numpy(), rtol=1e-4, atol=2e-4), n
         assert np.allclose(got, M.features(A), rtol=1e-4, atol=2e-4), n
+
+
+@pytest.mark.parametrize("n", [64, 50])
+def test_direct_bucket_takes_the_float64_tier(tsdx, n):
+    """A size bucket marked direct skips the network: every matrix goes to
+    the float64 tier (tier 2, stat -1) and still meets the bar; info as the
+    plain posv for a matrix that is not SPD."""
+    import torch
+    from tinysolverdx import router
+    rng = np.random.default_rng(12 + n)
+    A, _ = _spd(rng, 40, n, 0, 12)
+    A[5] = -np.eye(n)
+    B = rng.standard_normal((40, n, 1))
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    r = router.Router(router.init_params(0), direct=[64])
+    X = torch.zeros_like(dB)
+    info = torch.full((40,), -9, dtype=torch.int32, device="cuda")
+    stat = torch.full((40,), -9, dtype=torch.int32, device="cuda")
+    tier = torch.full((40,), -9, dtype=torch.int32, device="cuda")
+    r.posv(dA, dB, X, info=info, stat=stat, tier=tier)
+    torch.cuda.synchronize()
+    info, stat, tier = info.cpu().numpy(), stat.cpu().numpy(), tier.cpu().numpy()
+    assert (tier == 2).all() and (stat == -1).all()
+    assert info[5] == 1 and (np.delete(info, 5) == 0).all()
+    ok = info == 0
+    assert _bwd(tsdx, A[ok], X.cpu().numpy()[ok], B[ok]).max() <= 1e-14

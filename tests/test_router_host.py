@@ -1,0 +1,38 @@
+"""Host logic of the learned router (no GPU): size buckets, the direct
+float64 buckets and their save/load beside the parameters."""
+import numpy as np
+import pytest
+
+
+def test_size_buckets():
+    from tinysolverdx import router
+    assert [router.bucket(n) for n in (1, 4, 5, 8, 9, 16, 17, 32, 33, 48, 64)] == [4, 4, 8, 8, 16, 16, 32, 32, 64, 64, 64]
+
+
+def test_direct_buckets_round_trip(tmp_path):
+    from tinysolverdx import router
+    r = router.Router(router.init_params(3), device="cpu", direct=[32, 16])
+    assert r.direct == {16, 32}
+    path = tmp_path / "r.npy"
+    r.save(path, {"note": "test"})
+    q = router.Router.load(path, device="cpu")
+    assert q.direct == {16, 32}
+    assert np.array_equal(q.params.numpy(), r.params.numpy())
+    (tmp_path / "r.json").unlink()  # parameters alone: the network for every size
+    assert router.Router.load(path, device="cpu").direct == frozenset()
+
+
+def test_direct_buckets_validated():
+    from tinysolverdx import router
+    with pytest.raises(ValueError):
+        router.Router(router.init_params(0), device="cpu", direct=[24])
+
+
+def test_shipped_router_metadata():
+    """The shipped router's direct buckets came from timing on the GPU; they
+    are a subset of the buckets and stored with the training report."""
+    import json
+    from tinysolverdx import router
+    meta = json.loads(router.DEFAULT_PARAMS.with_suffix(".json").read_text())
+    assert set(meta["direct_buckets"]) <= set(router.BUCKETS)
+    assert "cost_ns_per_matrix" in meta["training"]

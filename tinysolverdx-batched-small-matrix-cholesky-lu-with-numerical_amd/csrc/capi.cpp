@@ -154,7 +154,7 @@ void posv_ir(const double* A, int n, int64_t batch, const double* B, double* X, 
              double tol, int* info, int* stat, void* work, size_t work_bytes, hipStream_t s) {
   check_dims(n, batch);
   if (nrhs < 0) throw InvalidArg("negative nrhs");
-  if (factor < TSDX_HIP_FACTOR_AUTO || factor > TSDX_HIP_FACTOR_FP64) throw InvalidArg("unknown factor precision");
+  if (factor < TSDX_HIP_FACTOR_AUTO || factor > TSDX_HIP_FACTOR_FP64_BLOCK) throw InvalidArg("unknown factor precision");
   if (max_iter < 0 || max_iter > 1000) throw InvalidArg("max_iter must be in [0, 1000]");
   if (!(tol > 0.0)) throw InvalidArg("tol must be positive");
   // one workgroup per matrix and int entries in the fallback list

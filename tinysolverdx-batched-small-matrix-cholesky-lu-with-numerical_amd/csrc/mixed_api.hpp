@@ -25,7 +25,8 @@ struct IrArgs {
   const int* lcount = nullptr;
 };
 
-// low: 1 float, 2 bfloat16, 3 half (refinement), 4 float64 direct for all.
+// low: 1 float, 2 bfloat16, 3 half (refinement), 4 plain float64 for all, 5
+// the float64 fallback tier for all.
 hipError_t launch_posv_ir(const IrArgs& a, int low, hipStream_t s);
 // float64 posv of the matrices on a.flist[0 .. *a.fcount).
 hipError_t launch_posv_gather(const IrArgs& a, hipStream_t s);

@@ -69,9 +69,9 @@ hipError_t launch_posv_gather(const IrArgs& a, hipStream_t s) {
 template <int NB>
 static hipError_t launch_ir(const IrArgs& a, int low, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, a.fcount);
-  if (low != 4) {
+  if (low != 4 && low != 5) {
     launch_mx<(NB + 15) / 16>(a, low, s);
-  } else {  // float64 direct for every matrix: the list is 0..batch-1
+  } else {  // float64 for every matrix (4 plain, 5 the fallback tier): the list is 0..batch-1
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((a.batch + 255) / 256)), dim3(256), 0, s, a.flist, a.fcount, a.stat, a.batch);
   }
   return gather<NB>(a, s, low == 4);

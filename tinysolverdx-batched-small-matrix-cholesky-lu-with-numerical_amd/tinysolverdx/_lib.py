@@ -58,7 +58,7 @@ _RES = {"tsdx_hip_last_error": C.c_char_p, "tsdx_hip_posv_ir_workspace_size": C.
 
 KIND = {"cholesky": 0, "lu": 1}
 
-FACTOR = {"auto": 0, "fp32": 1, "bf16": 2, "fp16": 3, "fp64": 4}
+FACTOR = {"auto": 0, "fp32": 1, "bf16": 2, "fp16": 3, "fp64": 4, "fp64_block": 5}
 
 _lib = None
 

@@ -124,9 +124,11 @@ def posv_ir(A, B, X, factor="fp32", max_iter=10, tol=1e-14, info=None, stat=None
     """Mixed-precision posv with float64 iterative refinement (float64 A, B, X).
 
     factor: "fp32", "bf16", "fp16" (low-precision factor + refinement), "fp64"
-    (plain float64 solve for every matrix) or "auto".  Matrices that do not
-    reach tol by refinement are solved again in float64.  stat (int32) gets
-    the corrections used per matrix, -1 for the float64 fallback."""
+    (plain float64 solve for every matrix), "fp64_block" (the float64
+    fallback tier for every matrix: block Cholesky on the matrix cores with
+    float64 refinement for n > 32) or "auto".  Matrices that do not reach tol
+    by refinement are solved again in float64.  stat (int32) gets the
+    corrections used per matrix, -1 for the float64 fallback."""
     import torch
     batch, n = _shape(A)
     if A.dtype != torch.float64:

@@ -17,6 +17,14 @@ solve as well).
 The routed solve (Router.posv) never trusts the network for accuracy: every
 answer is judged by the float64 residual, refinement that stalls falls back
 to the float64 solve, exactly as tsdx_hip_posv_ir_f64.
+
+A router also carries the sizes it solves directly in float64 without
+consulting the network (Router.direct, size buckets 4/8/16/32/64): after
+training, train_default times the routed solve against the float64 tier for
+every matrix on separately seeded matrices of the training distribution, and
+a bucket where routing does not pay (small n: the features and the network
+cost more than the whole float64 solve there) is marked direct.  The choice is
+stored next to the parameters (router_default.json).
 """
 import numpy as np
 
@@ -62,7 +70,8 @@ def measure_costs(n, count=32768, max_iter=10, tol=1e-14, seed=1):
     """Time per matrix (seconds) of each tier at size n, at throughput: the
     refinement tiers on well-conditioned matrices (kappa up to 1e3, so they
     converge and the float64 fallback stays empty), the float64 tier with the
-    plain posv.  Returns {0: bf16, 1: fp32, 2: fp64}."""
+    solve that tier-2 matrices and refinement failures actually take
+    (posv_ir factor "fp64_block").  Returns {0: bf16, 1: fp32, 2: fp64}."""
     import torch
     from . import device, matrices
     A, _ = matrices.spd_device(n, count, 1.0, 1e3, seed)
@@ -73,7 +82,8 @@ def measure_costs(n, count=32768, max_iter=10, tol=1e-14, seed=1):
     work = device.posv_ir_workspace(count)
     cost = {t: _timed(lambda: device.posv_ir(A, B, X, factor=f, max_iter=max_iter, tol=tol, work=work)) / count
             for t, f in ((0, "bf16"), (1, "fp32"))}
-    cost[2] = _timed(lambda: device.posv_solve(A, B, X, info)) / count
+    cost[2] = _timed(lambda: device.posv_ir(A, B, X, factor="fp64_block", max_iter=max_iter, tol=tol,
+                                            info=info, work=work)) / count
     return cost
 
 
@@ -99,6 +109,42 @@ def make_labels(A, B, cost, max_iter=10, tol=1e-14):
 
 
 DEFAULT_PARAMS = __import__("pathlib").Path(__file__).resolve().parent / "router_default.npy"
+BUCKETS = (4, 8, 16, 32, 64)
+
+
+def bucket(n):
+    """Size bucket of n (the smallest of 4, 8, 16, 32, 64 that holds it; 64
+    for larger n, which the library then rejects)."""
+    return next((b for b in BUCKETS if n <= b), BUCKETS[-1])
+
+
+def choose_direct(r, sizes=(16, 32, 64), count=65536, kmax=1e12, seed=20261210, reps=3):
+    """Size buckets where the float64 tier for every matrix is faster than the
+    routed solve, timed on `count` SPD matrices per size with kappa
+    log-uniform on [1, kmax] (the training distribution, fresh seed).
+    Returns (direct buckets, {n: {"routed": s, "fp64": s}} per matrix)."""
+    import torch
+    from . import device, matrices
+    direct, times = [], {}
+    saved = r.direct
+    r.direct = frozenset()
+    try:
+        for n in sizes:
+            A, _ = matrices.spd_device(n, count, 1.0, kmax, seed + n)
+            B = torch.randn((count, n, 1), generator=torch.Generator(device="cuda").manual_seed(n), device="cuda",
+                            dtype=torch.float64)
+            X = torch.empty_like(B)
+            wr = torch.empty(_lib.lib().tsdx_hip_posv_routed_workspace_size(count), dtype=torch.uint8, device="cuda")
+            wi = device.posv_ir_workspace(count)
+            t_r = _timed(lambda: r.posv(A, B, X, work=wr), reps) / count
+            t_d = _timed(lambda: device.posv_ir(A, B, X, factor="fp64_block", work=wi), reps) / count
+            times[n] = {"routed": t_r, "fp64": t_d}
+            if t_d <= t_r:
+                direct.append(bucket(n))
+            del A, B, X
+    finally:
+        r.direct = saved
+    return sorted(direct), times
 
 
 def training_set(count_per_n=20000, sizes=(16, 32, 64), kmax=1e12, seed=20261207, device="cuda"):
@@ -127,12 +173,17 @@ def train_default(path=DEFAULT_PARAMS, count_per_n=20000, epochs=60, seed=202612
     feat, label = training_set(count_per_n, seed=seed)
     r = Router(seed=seed)
     hist = r.fit(feat, label, epochs=epochs, batch=4096, lr=3e-3, seed=seed)
-    r.save(path)
+    direct, times = choose_direct(r)
+    r.direct = frozenset(direct)
     counts = np.bincount(label.cpu().numpy(), minlength=3).tolist()
     acc = float((r.predict(feat) == label).float().mean().item())
     costs = {n: {t: round(c * 1e9, 3) for t, c in v.items()} for n, v in training_set.costs.items()}
-    return {"loss_first": hist[0], "loss_last": hist[-1], "label_counts": counts, "train_accuracy": acc,
-            "cost_ns_per_matrix": costs}
+    report = {"loss_first": hist[0], "loss_last": hist[-1], "label_counts": counts, "train_accuracy": acc,
+              "cost_ns_per_matrix": costs, "direct_buckets": direct,
+              "routed_vs_fp64_ns_per_matrix": {n: {k: round(v * 1e9, 3) for k, v in t.items()}
+                                               for n, t in times.items()}}
+    r.save(path, report)
+    return report
 
 
 def default():
@@ -156,12 +207,16 @@ def init_params(seed=0):
 class Router:
     """Parameters live in a float32 device tensor of PARAMS entries."""
 
-    def __init__(self, params=None, device="cuda", seed=0):
+    def __init__(self, params=None, device="cuda", seed=0, direct=()):
         import torch
         p = init_params(seed) if params is None else np.asarray(params, np.float32)
         if p.shape != (PARAMS,):
             raise ValueError(f"router parameters must have {PARAMS} entries")
         self.params = torch.from_numpy(p.copy()).to(device)
+        bad = set(direct) - set(BUCKETS)
+        if bad:
+            raise ValueError(f"direct buckets must be among {BUCKETS}, got {sorted(bad)}")
+        self.direct = frozenset(direct)  # size buckets solved in float64 without the network
 
     # ---------------------------------------------------------------- inference
     def forward(self, feat):
@@ -224,19 +279,41 @@ class Router:
             history.append(float(torch.cat(losses).mean().item()))
         return history
 
-    def save(self, path):
+    def save(self, path, report=None):
+        """Parameters to `path` (.npy); the direct buckets (and the training
+        report, if given) to the .json beside it."""
+        import json
+        import pathlib
         np.save(path, self.params.cpu().numpy())
+        meta = {"direct_buckets": sorted(self.direct)}
+        if report is not None:
+            meta["training"] = report
+        pathlib.Path(path).with_suffix(".json").write_text(json.dumps(meta, indent=1, default=str) + "\n")
 
     @classmethod
     def load(cls, path, device="cuda"):
-        return cls(np.load(path, allow_pickle=False), device=device)
+        import json
+        import pathlib
+        meta = pathlib.Path(path).with_suffix(".json")
+        direct = json.loads(meta.read_text()).get("direct_buckets", []) if meta.exists() else []
+        return cls(np.load(path, allow_pickle=False), device=device, direct=direct)
 
     # -------------------------------------------------------------- routed posv
     def posv(self, A, B, X, max_iter=10, tol=1e-14, info=None, stat=None, tier=None, feat=None, work=None):
         """Routed posv (float64): X := A^-1 B; stat / tier / feat as documented
-        in include/tsdx_hip.h (tsdx_hip_posv_routed_f64)."""
+        in include/tsdx_hip.h (tsdx_hip_posv_routed_f64).  For a size in
+        self.direct every matrix takes the float64 tier without the network
+        (tier 2, stat -1, feat left untouched)."""
         import torch
         batch, n = _shape(A)
+        if bucket(n) in self.direct:
+            from . import device
+            wi = work if work is not None and work.numel() >= _lib.lib().tsdx_hip_posv_ir_workspace_size(batch) \
+                else None
+            device.posv_ir(A, B, X, factor="fp64_block", max_iter=max_iter, tol=tol, info=info, stat=stat, work=wi)
+            if tier is not None:
+                tier.fill_(2)
+            return X, info, stat, tier
         if work is None:
             work = torch.empty(_lib.lib().tsdx_hip_posv_routed_workspace_size(batch), dtype=torch.uint8,
                                device=A.device)
