@@ -119,15 +119,15 @@ def bucket(n):
 
 
 def choose_direct(r, sizes=(16, 32, 64), count=65536, kmax=1e12, seed=20261210, reps=3):
-    """Size buckets where the float64 tier for every matrix is faster than the
-    routed solve, timed on `count` SPD matrices per size with kappa
-    log-uniform on [1, kmax] (the training distribution, fresh seed).
-    Returns (direct buckets, {n: {"routed": s, "fp64": s}} per matrix)."""
+    """Size buckets where Router.posv is faster solving every matrix in
+    float64 (the direct path) than routing, timed both ways on `count` SPD
+    matrices per size with kappa log-uniform on [1, kmax] (the training
+    distribution, fresh seed).  Returns (direct buckets, {n: {"routed": s,
+    "fp64": s}} per matrix)."""
     import torch
-    from . import device, matrices
+    from . import matrices
     direct, times = [], {}
     saved = r.direct
-    r.direct = frozenset()
     try:
         for n in sizes:
             A, _ = matrices.spd_device(n, count, 1.0, kmax, seed + n)
@@ -135,9 +135,10 @@ def choose_direct(r, sizes=(16, 32, 64), count=65536, kmax=1e12, seed=20261210, 
                             dtype=torch.float64)
             X = torch.empty_like(B)
             wr = torch.empty(_lib.lib().tsdx_hip_posv_routed_workspace_size(count), dtype=torch.uint8, device="cuda")
-            wi = device.posv_ir_workspace(count)
+            r.direct = frozenset()
             t_r = _timed(lambda: r.posv(A, B, X, work=wr), reps) / count
-            t_d = _timed(lambda: device.posv_ir(A, B, X, factor="fp64_block", work=wi), reps) / count
+            r.direct = frozenset([bucket(n)])
+            t_d = _timed(lambda: r.posv(A, B, X, work=wr), reps) / count
             times[n] = {"routed": t_r, "fp64": t_d}
             if t_d <= t_r:
                 direct.append(bucket(n))
@@ -308,9 +309,15 @@ class Router:
         batch, n = _shape(A)
         if bucket(n) in self.direct:
             from . import device
-            wi = work if work is not None and work.numel() >= _lib.lib().tsdx_hip_posv_ir_workspace_size(batch) \
-                else None
-            device.posv_ir(A, B, X, factor="fp64_block", max_iter=max_iter, tol=tol, info=info, stat=stat, work=wi)
+            if n <= 32:  # the float64 tier there is the plain float64 posv: one launch, no list
+                device.posv_solve(A, B, X, info)
+                if stat is not None:
+                    stat.fill_(-1)
+            else:
+                wi = work if work is not None and work.numel() >= _lib.lib().tsdx_hip_posv_ir_workspace_size(batch) \
+                    else None
+                device.posv_ir(A, B, X, factor="fp64_block", max_iter=max_iter, tol=tol, info=info, stat=stat,
+                               work=wi)
             if tier is not None:
                 tier.fill_(2)
             return X, info, stat, tier
