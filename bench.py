@@ -551,10 +551,12 @@ def run_router(a):
     R = rt.Router(seed=3)
     hist = R.fit(feat, label, epochs=60, batch=4096, lr=3e-3, seed=3)
     train_acc = float((R.predict(feat) == label).float().mean().item())
+    shift = R.calibrate(rt.training_set.costs[64])  # expected-cost decision between fp32 and float64
     direct, dtimes = rt.choose_direct(R)  # size buckets where the network does not pay (timed, fresh seed)
     R.direct = frozenset(direct)
     log(f"[bench] router trained in {time.perf_counter() - t0:.1f}s: loss {hist[0]:.3f} -> {hist[-1]:.3f}, "
         f"labels {np.bincount(label.cpu().numpy(), minlength=3).tolist()}, train accuracy {train_acc:.3f}, "
+        f"fp64 logit shift {shift:.3f}, "
         f"direct float64 buckets {direct} (routed vs fp64 ns/matrix "
         f"{ {n: {k: round(v * 1e9, 2) for k, v in t.items()} for n, t in dtimes.items()} })")
     sizes = (16, 32, 64)
@@ -599,7 +601,7 @@ def run_router(a):
     cfg = {"workload": f"routed posv, {count} SPD fp64 matrices, n uniform on {{16,32,64}}, kappa log-uniform "
                        "[1,1e12], nrhs=1; router trained in this run on 3 x 20000 separately seeded matrices",
            "train_loss_first_last": [round(hist[0], 4), round(hist[-1], 4)], "train_accuracy": round(train_acc, 4),
-           "direct_float64_buckets": direct,
+           "direct_float64_buckets": direct, "fp64_logit_shift": round(shift, 4),
            "train_labels_bf16_fp32_fp64": np.bincount(label.cpu().numpy(), minlength=3).tolist(),
            "routed_tiers_bf16_fp32_fp64": tiers.tolist(),
            "unrouted_fp32_solves_per_s": round(rate["unrouted"], 1), "plain_fp64_solves_per_s": round(rate["fp64"], 1),

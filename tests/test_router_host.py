@@ -36,3 +36,17 @@ def test_shipped_router_metadata():
     meta = json.loads(router.DEFAULT_PARAMS.with_suffix(".json").read_text())
     assert set(meta["direct_buckets"]) <= set(router.BUCKETS)
     assert "cost_ns_per_matrix" in meta["training"]
+
+
+def test_calibrate_shifts_the_float64_logit():
+    """theta = (c2 - c1) / c2: with float64 three times the fp32 cost the
+    float64 tier must win at p > 2/3, i.e. its logit moves by -log 2."""
+    from tinysolverdx import router
+    r = router.Router(router.init_params(1), device="cpu")
+    before = r.params.numpy().copy()
+    shift = r.calibrate({0: 9.0, 1: 1.0, 2: 3.0})
+    after = r.params.numpy()
+    assert abs(shift + np.log(2.0)) < 1e-6
+    assert abs(after[router.OFF_B3 + 2] - before[router.OFF_B3 + 2] - shift) < 1e-6
+    diff = np.flatnonzero(after != before)
+    assert list(diff) == [router.OFF_B3 + 2]

@@ -174,13 +174,14 @@ def train_default(path=DEFAULT_PARAMS, count_per_n=20000, epochs=60, seed=202612
     feat, label = training_set(count_per_n, seed=seed)
     r = Router(seed=seed)
     hist = r.fit(feat, label, epochs=epochs, batch=4096, lr=3e-3, seed=seed)
+    acc = float((r.predict(feat) == label).float().mean().item())
+    shift = r.calibrate(training_set.costs[max(training_set.costs)])  # the largest size: where the network runs
     direct, times = choose_direct(r)
     r.direct = frozenset(direct)
     counts = np.bincount(label.cpu().numpy(), minlength=3).tolist()
-    acc = float((r.predict(feat) == label).float().mean().item())
     costs = {n: {t: round(c * 1e9, 3) for t, c in v.items()} for n, v in training_set.costs.items()}
     report = {"loss_first": hist[0], "loss_last": hist[-1], "label_counts": counts, "train_accuracy": acc,
-              "cost_ns_per_matrix": costs, "direct_buckets": direct,
+              "fp64_logit_shift": round(shift, 4), "cost_ns_per_matrix": costs, "direct_buckets": direct,
               "routed_vs_fp64_ns_per_matrix": {n: {k: round(v * 1e9, 3) for k, v in t.items()}
                                                for n, t in times.items()}}
     r.save(path, report)
@@ -246,6 +247,21 @@ class Router:
                   _chk(label, "label", torch.int32), count, _chk(g, "grad"), _chk(loss, "loss"), _chk(work, "work"),
                   work.numel(), _stream(feat))
         return g, loss
+
+    def calibrate(self, cost):
+        """Expected-cost decision between the fp32 and float64 tiers.  The
+        network estimates p = P(fp32 refinement fails | features); under the
+        label cost model (a failed refinement pays c1 and then c2) choosing
+        float64 is cheaper iff p > theta = (c2 - c1) / c2, while the argmax of
+        the logits decides at p > 1/2.  Shifting the float64 logit by
+        -log(theta / (1 - theta)) makes the argmax take the expected-cost
+        decision.  cost: {tier: cost} at the size the network is used for.
+        Returns the shift applied to the float64 logit's bias."""
+        c1, c2 = float(cost[1]), float(cost[2])
+        theta = min(max((c2 - c1) / c2, 0.05), 0.95) if c2 > 0 else 0.5
+        shift = -float(np.log(theta / (1.0 - theta)))
+        self.params[OFF_B3 + 2] += shift
+        return shift
 
     def set_scaling(self, feat):
         """Feature means and scales from a training set."""
