@@ -153,6 +153,37 @@ def test_forward_pass_is_bit_exact(hip, oracle):
             assert bits_equal(H.lu_solve(LU, piv, B), oracle.lu_solve(LU, piv, B)), (dt, n)
 
 
+@pytest.mark.parametrize("n", [33, 40, 64])
+def test_getrf_multirow_batches_bit_exact(hip, oracle, n):
+    """fp32 getrf batches of >= 16384 matrices at 32 < n <= 64 run the
+    two-rows-per-lane kernel (k_getrf_rows); factors, pivots and info must be
+    the reference's, bit for bit, like the one-row kernel's: zero columns,
+    a zero matrix, duplicate rows, NaN and inf entries and ties included."""
+    import torch
+    from tinysolverdx import device
+    rng = np.random.default_rng(1000 + n)
+    batch = 16384
+    G = rng.standard_normal((batch, n, n)).astype(np.float32)
+    zc = rng.choice(batch, 160, replace=False)
+    G[zc, :, rng.integers(0, n, 160)] = 0.0
+    G[7] = 0.0
+    G[11, 5] = G[11, 2]
+    G[13, 3, 4] = np.nan
+    G[17, :, 0] = 1.0  # ties in the first column: the first row wins
+    G[19, 9, 9] = np.inf
+    G[23, n - 1] = -G[23, n - 1]
+    print(f"[getrf multirow] n={n} batch={batch}", flush=True)
+    dG = torch.from_numpy(G).cuda()
+    piv = torch.zeros((batch, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    device.getrf(dG, piv, info)
+    LUo, pivo, io = oracle.lu(G)
+    torch.cuda.synchronize()
+    assert np.array_equal(info.cpu().numpy(), io)
+    assert np.array_equal(piv.cpu().numpy(), pivo)
+    assert bits_equal(dG.cpu().numpy(), LUo)
+
+
 # ------------------------------------------------------------ device entry points
 def test_device_entry_points_and_unaligned_rows(hip, oracle):
     import torch

@@ -33,7 +33,7 @@ namespace {
 
 constexpr int kClasses = 3;
 
-int nvariants(bool dbl, int cls) { return cls == 0 ? 3 : (cls == 1 ? 2 : (dbl ? 1 : 2)); }
+int nvariants(bool dbl, int cls) { return cls == 0 ? 3 : (cls == 1 ? 2 : (dbl ? 1 : 3)); }
 
 // Type-erased access to the compiled (class, variant) launchers.
 template <class T>
@@ -53,7 +53,7 @@ template <>
 VariantOps<float> variant_ops<float>(int cls, int var) {
   if (cls == 0) return var == 0 ? ops_of<float, 0, 0>() : var == 1 ? ops_of<float, 0, 1>() : ops_of<float, 0, 2>();
   if (cls == 1) return var == 0 ? ops_of<float, 1, 0>() : ops_of<float, 1, 1>();
-  return var == 0 ? ops_of<float, 2, 0>() : ops_of<float, 2, 1>();
+  return var == 0 ? ops_of<float, 2, 0>() : var == 1 ? ops_of<float, 2, 1>() : ops_of<float, 2, 2>();
 }
 template <>
 VariantOps<double> variant_ops<double>(int cls, int var) {

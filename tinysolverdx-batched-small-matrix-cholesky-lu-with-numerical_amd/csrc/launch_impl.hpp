@@ -31,17 +31,44 @@ hipError_t launch_potrf(T* A, int n, int64_t batch, int* info, int vec, hipStrea
   return hipGetLastError();
 }
 
+// Below this batch the one-row LU kernel is used even where a multi-row shape
+// exists: with fewer matrices than a few rounds of waves the longer wave of
+// the multi-row kernel costs more than its throughput gains (fp32 n=64 on
+// MI355X: 89 vs 67 us at 1024 matrices, 216 vs 199 us at 8192, 350 vs 356 us
+// at 16384, 1175 vs 1291 us at 65536; both give identical results).
+constexpr int64_t kGetrfRowsMinBatch = 16384;
+
+template <class T, int NB, bool Z>
+static void getrf_go(T* A, int n, int64_t batch, int* piv, int* info, int vec, hipStream_t s) {
+  using S = GetrfShape<T, NB>;
+  auto one_row = [&] {
+    hipLaunchKernelGGL((k_getrf<T, NB, Z>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
+  };
+  if constexpr (S::R == 1) {
+    one_row();
+  } else {
+    if (batch < kGetrfRowsMinBatch) {
+      one_row();
+      return;
+    }
+    constexpr int64_t per_block = (int64_t)S::WPB * (kWave / S::L);
+    const dim3 grid((unsigned)((batch + per_block - 1) / per_block));
+    hipLaunchKernelGGL((k_getrf_rows<T, S::L, S::R, S::WPB, Z>), grid, dim3(kWave * S::WPB), 0, s, A, n, batch, piv,
+                       info, vec);
+  }
+}
+
 template <class T, int NB>
 hipError_t launch_getrf(T* A, int n, int64_t batch, int* piv, int* info, int vec, bool ztile, hipStream_t s) {
   if constexpr (NB <= 16) {  // n <= 16: always the zero-pivot tile rule
-    hipLaunchKernelGGL((k_getrf<T, NB, true>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
+    getrf_go<T, NB, true>(A, n, batch, piv, info, vec, s);
   } else if constexpr (NB == 64) {  // n >= 33: always the skip rule
-    hipLaunchKernelGGL((k_getrf<T, NB, false>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
+    getrf_go<T, NB, false>(A, n, batch, piv, info, vec, s);
   } else {
     if (ztile)
-      hipLaunchKernelGGL((k_getrf<T, NB, true>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
+      getrf_go<T, NB, true>(A, n, batch, piv, info, vec, s);
     else
-      hipLaunchKernelGGL((k_getrf<T, NB, false>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
+      getrf_go<T, NB, false>(A, n, batch, piv, info, vec, s);
   }
   return hipGetLastError();
 }

@@ -11,7 +11,7 @@
 // for the registers of large ones - pull tasks from their own atomic counter.
 //
 // A task runs the same per-matrix code as the per-size kernels (chol_rows,
-// lu_regs): the steps run to the largest size in the task, and steps past a
+// lu_regs, lu_rows): the steps run to the largest size in the task, and steps past a
 // matrix's own size only touch its zero padding (info is counted per matrix),
 // so every factor, pivot and info is bit-identical to a per-size launch.  LU
 // keeps the reference's zero-pivot rule per matrix (divide by 1 for n <= 24,
@@ -62,27 +62,30 @@ constexpr int rbucket_nb(int code) {
 }
 
 // Compiled variants per (type, class): waves per workgroup and rows per lane of
-// the Cholesky buckets (matrices per wave = 64 / (NB / R)).  The autotuner
-// times the variants of each class on the plan's size mix and keeps the best.
+// the Cholesky buckets (matrices per wave = 64 / (NB / R)), and of the LU
+// bucket n <= 64 (RL64).  The autotuner times the variants of each class on
+// the plan's size mix and keeps the best.
 template <class T, int CLS, int VAR>
 struct RaggedVariant;
 // class 0 (n <= 16)
-template <class T> struct RaggedVariant<T, 0, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1; };
-template <class T> struct RaggedVariant<T, 0, 1> { static constexpr int WPB = 4, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1; };
-template <class T> struct RaggedVariant<T, 0, 2> { static constexpr int WPB = 1, R4 = 1, R8 = 2, R16 = 4, R32 = 1, R64 = 1; };
+template <class T> struct RaggedVariant<T, 0, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1; };
+template <class T> struct RaggedVariant<T, 0, 1> { static constexpr int WPB = 4, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1; };
+template <class T> struct RaggedVariant<T, 0, 2> { static constexpr int WPB = 1, R4 = 1, R8 = 2, R16 = 4, R32 = 1, R64 = 1, RL64 = 1; };
 // class 1 (n <= 32)
-template <class T> struct RaggedVariant<T, 1, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1; };
-template <class T> struct RaggedVariant<T, 1, 1> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 4, R64 = 1; };
+template <class T> struct RaggedVariant<T, 1, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1; };
+template <class T> struct RaggedVariant<T, 1, 1> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 4, R64 = 1, RL64 = 1; };
 // class 2 (n <= 64); two rows per lane only for float (registers)
-template <class T> struct RaggedVariant<T, 2, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1; };
-template <class T> struct RaggedVariant<T, 2, 1> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 2; };
+template <class T> struct RaggedVariant<T, 2, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1; };
+template <class T> struct RaggedVariant<T, 2, 1> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 2, RL64 = 1; };
+template <class T> struct RaggedVariant<T, 2, 2> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 2, RL64 = 2; };
 
 // Matrices per wave task for a bucket under a variant.
 template <class V>
 constexpr int rtask_capacity(int code) {
   const int nb = rbucket_nb(code);
   const bool chol = code <= RC_CHOL64;
-  const int r = !chol ? 1 : nb == 4 ? V::R4 : nb == 8 ? V::R8 : nb == 16 ? V::R16 : nb == 32 ? V::R32 : V::R64;
+  const int r = !chol ? (nb == 64 ? V::RL64 : 1)
+                      : nb == 4 ? V::R4 : nb == 8 ? V::R8 : nb == 16 ? V::R16 : nb == 32 ? V::R32 : V::R64;
   return kWave / (nb / r);
 }
 
@@ -140,6 +143,29 @@ __device__ __forceinline__ void ragged_lu(const RaggedArgs<T>& a, const RTask& t
   if (x.live && i == 0 && a.info) a.info[x.m] = inf;
 }
 
+// LU of one task, L lanes x R rows per matrix (as k_getrf_rows).
+template <class T, int L, int R, bool ZTILE>
+__device__ __forceinline__ void ragged_lu_rows(const RaggedArgs<T>& a, const RTask& t, int lane, T* rowb) {
+  constexpr int N = L * R;
+  const int g = lane / L, q = lane % L;
+  const RMat<T> x = rmat(a, t, g);
+  T r[R][N];
+  static_for<0, R>([&](auto s) {
+    const int row = q + L * s;
+    load_row<T, N>(r[s], x.A + (int64_t)row * x.n, x.n, x.live && row < x.n, x.vec);
+  });
+  int pos[R], mypiv[R], inf;
+  lu_rows<T, L, R, ZTILE>(r, t.nmax, x.n, q, g, rowb, pos, mypiv, inf);
+  static_for<0, R>([&](auto s) {
+    const int row = q + L * s;
+    if (x.live && row < x.n) {
+      store_row<T, N>(r[s], x.A + (int64_t)pos[s] * x.n, x.n, x.vec, false, 0);
+      if (a.piv) a.piv[a.poff[x.m] + row] = mypiv[s];
+    }
+  });
+  if (x.live && q == 0 && a.info) a.info[x.m] = inf;
+}
+
 template <class V>
 constexpr int rvariant_lds_rows() {  // per-wave LDS in elements / (2 * 64)
   int r = 1;
@@ -148,6 +174,7 @@ constexpr int rvariant_lds_rows() {  // per-wave LDS in elements / (2 * 64)
   r = V::R16 > r ? V::R16 : r;
   r = V::R32 > r ? V::R32 : r;
   r = V::R64 > r ? V::R64 : r;
+  r = V::RL64 > r ? V::RL64 : r;
   return r;
 }
 
@@ -186,7 +213,12 @@ __global__ __launch_bounds__((kWave * RaggedVariant<T, CLS, VAR>::WPB)) void k_r
     } else {
       switch (code) {
         case RC_CHOL64: ragged_chol<T, 64 / V::R64, V::R64>(a, tk, lane, scratch); break;
-        default: ragged_lu<T, 64, false>(a, tk, lane, scratch); break;
+        default:
+          if constexpr (V::RL64 == 1)
+            ragged_lu<T, 64, false>(a, tk, lane, scratch);
+          else
+            ragged_lu_rows<T, 64 / V::RL64, V::RL64, false>(a, tk, lane, scratch);
+          break;
       }
     }
   }

@@ -118,6 +118,78 @@ __device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int nloop, int n, int
   return info;
 }
 
+// LU with partial pivoting, L lanes x R rows per matrix (rows cyclic over
+// lanes as in chol_rows).  Rows never move: each row keeps its registers and
+// tracks its current position pos[s]; on exit row pos[s] of the packed L\U
+// factor is in r[s], and lane q holds piv[k] for k = q + L*s in mypiv[s].  The
+// per-element chain, the pivot choice (key = |a_ik|, NaN -> -1, a NaN at the
+// current position -> +inf, then the smallest position among the maxima) and
+// the zero-pivot rule (ZTILE: divide by 1, else skip the column) are those of
+// lu_regs (plain_kernels.hpp), so factors, pivots and info are identical.
+// What R rows per lane buy: the pivot search reduces over the lane's R rows
+// first and over L lanes after, and one refined reciprocal of the pivot
+// gives the lane's R quotients (ColumnDivider: correctly rounded).
+// rowb: per-wave LDS, 2 * 64 * R elements.
+template <class T, int L, int R, bool ZTILE>
+__device__ __forceinline__ void lu_rows(T (&r)[R][L * R], int nloop, int n, int q, int g, T* rowb, int (&pos)[R],
+                                        int (&mypiv)[R], int& info) {
+  constexpr int N = L * R;
+  info = 0;
+  bool act[R];
+  static_for<0, R>([&](auto s) {
+    pos[s] = q + L * s;
+    mypiv[s] = 0;
+    act[s] = q + L * s < n;
+  });
+  static_for<0, N>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k < nloop) {
+      T key[R];
+      T best = T(-2);
+      static_for<0, R>([&](auto s) {
+        const T v = absT(r[s][k]);
+        const T kk = (v != v) ? (pos[s] == k ? infT<T>() : T(-1)) : v;
+        key[s] = (act[s] && pos[s] >= k) ? kk : T(-2);
+        best = key[s] > best ? key[s] : best;
+      });
+      const T mx = group_allreduce<L>(best, [](T a, T b) { return a > b ? a : b; });
+      int cpl = 0x7fffffff;
+      static_for<0, R>([&](auto s) {
+        const bool c = act[s] && pos[s] >= k && key[s] == mx;
+        cpl = (c && pos[s] < cpl) ? pos[s] : cpl;
+      });
+      const int p = group_allreduce<L>(cpl, [](int a, int b) { return a < b ? a : b; });
+      if (q == k % L) mypiv[k / L] = p;
+      static_for<0, R>([&](auto s) {
+        const bool isp = act[s] && pos[s] == p;
+        pos[s] = (pos[s] == k) ? p : (isp ? k : pos[s]);
+      });
+      T* rb = rowb + (k & 1) * (kWave * R) + g * N;
+      static_for<0, R>([&](auto s) {
+        if (act[s] && pos[s] == k) static_for<k, N>([&](auto jc) { rb[jc] = r[s][jc]; });
+      });
+      wave_sync();
+      const T pk = rb[k];
+      const bool zero = (pk == T(0));
+      info = (zero && info == 0 && k < n) ? k + 1 : info;
+      if (ZTILE || !zero) {
+        T a[R], f[R];
+        static_for<0, R>([&](auto s) { a[s] = r[s][k]; });
+        ColumnDivider<T, R>::apply(a, zero ? T(1) : pk, [&](auto sl, T x) { f[decltype(sl)::value] = x; });
+        static_for<0, R>([&](auto s) {
+          if (act[s] && pos[s] > k) {
+            r[s][k] = f[s];
+            static_for<k + 1, N>([&](auto jc) {
+              constexpr int j = decltype(jc)::value;
+              r[s][j] = fmaT(-f[s], rb[j], r[s][j]);
+            });
+          }
+        });
+      }
+    }
+  });
+}
+
 // Group shape per (type, bucket): L lanes x R rows, WPB waves per workgroup.
 // R = 1 selects the one-row kernel of plain_kernels.hpp.  Measured on MI355X
 // (uniform batches, fp32 n=32 65536 matrices: 301 us one-row, 171 us 8x4).
@@ -166,6 +238,49 @@ __global__ __launch_bounds__(kWave * WPB) void k_potrf_rows(T* __restrict__ A, i
   };
   const int inf = chol_rows<T, L, R>(r, n, n, q, g, col_s[wv], store_slot);
   store_slot(ic<R - 1>{});
+  if (live && q == 0 && info) info[m] = inf;
+}
+
+// Group shape of the LU kernel per (type, bucket), as PotrfShape; R = 1
+// selects the one-row kernel of plain_kernels.hpp.  Measured on MI355X (fp32,
+// bit-identical results): n=64, 65536 matrices 1.321 ms one-row, 1.203 ms
+// 32 x 2; but n=32 (262144) 0.850 ms one-row vs 0.985 ms 8 x 4 and n=16
+// (2^20) 0.630 vs 0.773 ms 4 x 4: with few columns per step the per-row
+// quotients and the per-row masked updates cost more than the shared pivot
+// search saves, so the small buckets keep one row per lane.
+template <class T, int NB>
+struct GetrfShape {
+  static constexpr int L = NB, R = 1, WPB = waves_per_block<NB>();
+};
+template <>
+struct GetrfShape<float, 64> {
+  static constexpr int L = 32, R = 2, WPB = 1;
+};
+
+template <class T, int L, int R, int WPB, bool ZTILE>
+__global__ __launch_bounds__(kWave * WPB) void k_getrf_rows(T* __restrict__ A, int n, int64_t batch,
+                                                           int* __restrict__ piv, int* __restrict__ info, int vec) {
+  constexpr int N = L * R;
+  __shared__ T rowb_s[WPB][2 * kWave * R];
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  const int g = lane / L, q = lane % L;
+  const int64_t m = ((int64_t)blockIdx.x * WPB + wv) * (kWave / L) + g;
+  const bool live = m < batch;
+  const int64_t nn = (int64_t)n * n;
+  T r[R][N];
+  static_for<0, R>([&](auto s) {
+    const int row = q + L * s;
+    load_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, live && row < n, vec != 0);
+  });
+  int pos[R], mypiv[R], inf;
+  lu_rows<T, L, R, ZTILE>(r, n, n, q, g, rowb_s[wv], pos, mypiv, inf);
+  static_for<0, R>([&](auto s) {
+    const int row = q + L * s;
+    if (live && row < n) {
+      store_row<T, N>(r[s], A + m * nn + (int64_t)pos[s] * n, n, vec != 0, false, 0);
+      piv[m * n + row] = mypiv[s];
+    }
+  });
   if (live && q == 0 && info) info[m] = inf;
 }
 
