@@ -184,6 +184,30 @@ def test_getrf_multirow_batches_bit_exact(hip, oracle, n):
     assert bits_equal(dG.cpu().numpy(), LUo)
 
 
+@pytest.mark.parametrize("dt,n,batch", [(np.float32, 32, 65536), (np.float32, 13, 131072), (np.float32, 50, 16384),
+                                         (np.float64, 29, 32768)])
+def test_potrf_persistent_batches_bit_exact(hip, oracle, dt, n, batch):
+    """Batches of several rounds of waves of the multi-row Cholesky
+    (k_potrf_rows; fp32 n=32 is config 0): factors and info bit-exact against
+    the oracle, failing matrices (tile semantics) included."""
+    import torch
+    from tinysolverdx import device
+    rng = np.random.default_rng(2000 + n)
+    base = _spd(rng, 4096, n, dt)
+    S = np.concatenate([base] * (batch // 4096))
+    S[5] = -np.eye(n, dtype=dt)
+    S[batch - 3, n // 2, n // 2] = -1.0
+    S[batch // 2, 1, 0] = np.nan
+    print(f"[potrf persistent] {np.dtype(dt).name} n={n} batch={batch}", flush=True)
+    dS = torch.from_numpy(S).cuda()
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    device.potrf(dS, info)
+    Lo, io = oracle.cholesky(S)
+    torch.cuda.synchronize()
+    assert np.array_equal(info.cpu().numpy(), io)
+    assert bits_equal(dS.cpu().numpy(), Lo)
+
+
 # ------------------------------------------------------------ device entry points
 def test_device_entry_points_and_unaligned_rows(hip, oracle):
     import torch
