@@ -257,7 +257,10 @@ def run_posv(a):
                    "bwd_err_le_1e-14": f"{ok_m}/{batch}", "fp64_fallback": fallback,
                    "plain_fp64_solves_per_s": round(plain_rate, 1),
                    "plain_fp64_bwd_err_le_1e-14": f"{ok}/{batch}",
-                   "fixed_tier_over_plain": round(mixed_rate / plain_rate, 3)},
+                   "fixed_tier_over_plain": round(mixed_rate / plain_rate, 3),
+                   # the reference's convention (its bench.py): posv = n^3/3 + 2 n^2 nrhs
+                   "gflops_routed_fixed_plain": [round(r * (n ** 3 / 3 + 2 * n * n * nrhs) / 1e9, 1)
+                                                 for r in (routed_rate, mixed_rate, plain_rate)]},
         "roofline": roofline,
     }
     if a.copy_probe:
@@ -379,6 +382,7 @@ def run_potrf(a):
     return _line("potrf matrices/s fp32 n=32 (65536 SPD); HBM GB/s", batch / (ms * 1e-3), "matrices/s", a, ms,
                  "f32", {"workload": f"potrf fp32 n={n}, batch={batch}, Q diag(lambda) Q^T, lambda log-uniform "
                                      "[1e-3,1]", "info_gt0": bad, "max_rel_residual_sampled": res,
+                         "gflops": round(batch / (ms * 1e-3) * n ** 3 / 3 / 1e9, 1),
                          "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)},
                  _roof(per * batch, ms * 1e-3, "k_potrf_rows<float,8,4,1>", per), cpu)
 
@@ -420,12 +424,21 @@ def run_getrf(a):
         cpu = _cpu_timed(lambda: orc.lu(Ah), take, "matrices/s", f"first {take} of the {batch} matrices")
     cfg = {"workload": f"getrf {a.dtype} n={n}, batch={batch} (~1 GiB of A), N(0,1), 1% zeroed columns",
            "info_gt0": bad, "zeroed_columns": int(zc.numel()),
+           "gflops": round(batch / (ms * 1e-3) * 2 * n ** 3 / 3 / 1e9, 1),
            "gesv4_matrices_per_s": round(batch / (ms_s * 1e-3), 1),
+           "gesv4_gflops": round(batch / (ms_s * 1e-3) * (2 * n ** 3 / 3 + 2 * n * n * 4) / 1e9, 1),
            "gesv4_GBps": round(per_s * batch / (ms_s * 1e-3) / 1e9, 1),
            "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)}
     return _line(f"getrf matrices/s {a.dtype} n={n} (~1 GiB of A); HBM GB/s", batch / (ms * 1e-3), "matrices/s",
-                 a, ms, a.dtype, cfg, _roof(per * batch, ms * 1e-3, f"k_getrf<{'float' if es == 4 else 'double'},"
-                                                                    f"{max(4, 1 << (n - 1).bit_length())}>", per), cpu)
+                 a, ms, a.dtype, cfg, _roof(per * batch, ms * 1e-3, _getrf_kernel(es, n, batch), per), cpu)
+
+
+def _getrf_kernel(es, n, batch):
+    """Name of the kernel getrf launches for this element size, n and batch."""
+    nb = max(4, 1 << (n - 1).bit_length())
+    if es == 4 and nb == 64 and batch >= 16384:  # two rows per lane (csrc/rows_kernels.hpp, GetrfShape)
+        return "k_getrf_rows<float,32,2,1>"
+    return f"k_getrf<{'float' if es == 4 else 'double'},{nb}>"
 
 
 def ragged_inputs(count, dtype, seed):
@@ -529,6 +542,8 @@ def run_ragged(a):
                        "(kappa log-uniform [1,1e4]) half N(0,1) LU, one graph launch",
            "plan": {k: v for k, v in plan.report.items()}, "info_gt0": bad,
            "per_size_launches_ms": round(per_ms, 4), "graph_over_per_size": round(per_ms / ms, 3),
+           "gflops": round(float(((n64.astype(np.float64) ** 3) * np.where(kinds == 1, 2.0 / 3.0, 1.0 / 3.0)).sum())
+                           / (ms * 1e-3) / 1e9, 1),
            "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)}
     return _line(f"ragged factor matrices/s {a.dtype} (1M, n 2..64, half Cholesky half LU); HBM GB/s",
                  count / (ms * 1e-3), "matrices/s", a, ms, a.dtype, cfg,
