@@ -185,27 +185,37 @@ def test_getrf_multirow_batches_bit_exact(hip, oracle, n):
 
 
 @pytest.mark.parametrize("dt,n,batch", [(np.float32, 32, 65536), (np.float32, 13, 131072), (np.float32, 50, 16384),
-                                         (np.float64, 29, 32768)])
+                                         (np.float64, 29, 32768), (np.float32, 32, 65531), (np.float32, 16, 65539),
+                                         (np.float64, 32, 32771), (np.float32, 64, 16387)])
 def test_potrf_persistent_batches_bit_exact(hip, oracle, dt, n, batch):
     """Batches of several rounds of waves of the multi-row Cholesky
     (k_potrf_rows; fp32 n=32 is config 0): factors and info bit-exact against
-    the oracle, failing matrices (tile semantics) included."""
+    the oracle, failing matrices (tile semantics) included.  The full-bucket
+    sizes (16, 32, 64) take the line-ordered stores through LDS or the
+    lower-triangle loads; batches that are not a multiple of the matrices per
+    wave leave the last wave partly empty, whose absent matrices must not be
+    written."""
     import torch
     from tinysolverdx import device
     rng = np.random.default_rng(2000 + n)
     base = _spd(rng, 4096, n, dt)
-    S = np.concatenate([base] * (batch // 4096))
+    S = np.concatenate([base] * (batch // 4096 + 1))[:batch]
     S[5] = -np.eye(n, dtype=dt)
     S[batch - 3, n // 2, n // 2] = -1.0
     S[batch // 2, 1, 0] = np.nan
     print(f"[potrf persistent] {np.dtype(dt).name} n={n} batch={batch}", flush=True)
-    dS = torch.from_numpy(S).cuda()
+    # the batch sits between two guard regions that must come back untouched
+    buf = torch.full((batch + 2, n, n), 7.0, dtype=torch.float32 if dt == np.float32 else torch.float64,
+                     device="cuda")
+    dS = buf[1:batch + 1]
+    dS.copy_(torch.from_numpy(S))
     info = torch.zeros(batch, dtype=torch.int32, device="cuda")
     device.potrf(dS, info)
     Lo, io = oracle.cholesky(S)
     torch.cuda.synchronize()
     assert np.array_equal(info.cpu().numpy(), io)
     assert bits_equal(dS.cpu().numpy(), Lo)
+    assert bool((buf[0] == 7.0).all()) and bool((buf[batch + 1] == 7.0).all())
 
 
 # ------------------------------------------------------------ device entry points

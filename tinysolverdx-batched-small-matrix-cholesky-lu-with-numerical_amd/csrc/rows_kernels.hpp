@@ -20,6 +20,7 @@
 
 #include "device_common.hpp"
 
+
 namespace tsdx_hip {
 
 // Newton-refined reciprocal: v_rcp_f64 then two corrections.
@@ -64,7 +65,10 @@ struct ColumnDivider<double, R> {
 };
 
 // Factor in registers.  r[s][j] = A(q + L*s, j) on entry, L on exit (entries
-// above the diagonal are garbage).  col: per-wave LDS, 2 * 64 * R elements.
+// above the diagonal are garbage).  Columns j >= L*(s+1), right of every row
+// of slot s, are neither read nor written, so a caller that does not touch
+// them either leaves them to the compiler to drop: slot s then needs L*(s+1)
+// registers, not L*R.  col: per-wave LDS, 2 * 64 * R elements.
 // nloop / n as for chol_regs (plain_kernels.hpp).
 // done(ic<s>) is called as soon as slot s < R - 1 holds final rows (after the
 // pivot of row L*s + L - 1), so that a caller can store them while the
@@ -107,7 +111,10 @@ __device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int nloop, int n, int
       static_for<k + 2, N>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         const T c = cb[j];
-        static_for<SA, R>([&](auto s) { r[s][j] = fmaT(-r[s][k], c, r[s][j]); });
+        static_for<SA, R>([&](auto s) {
+          // columns right of slot s's last row hold nothing the factor needs
+          if constexpr (j < L * (decltype(s)::value + 1)) r[s][j] = fmaT(-r[s][k], c, r[s][j]);
+        });
       });
     }
     // rows L*s .. L*s + L - 1 are final once pivot(L*s + L - 1) has run (in
@@ -190,51 +197,147 @@ __device__ __forceinline__ void lu_rows(T (&r)[R][L * R], int nloop, int n, int 
   });
 }
 
-// Group shape per (type, bucket): L lanes x R rows, WPB waves per workgroup.
-// R = 1 selects the one-row kernel of plain_kernels.hpp.  Measured on MI355X
-// (uniform batches, fp32 n=32 65536 matrices: 301 us one-row, 171 us 8x4).
+// Group shape per (type, bucket): L lanes x R rows, WPB waves per workgroup;
+// R = 1 selects the one-row kernel of plain_kernels.hpp.  OCC: waves per SIMD
+// the register allocation must allow.  LOWER: load only up to the diagonal
+// (load_row_lower).  STAGE: store through LDS with whole-line writes
+// (store_slot_lines).  Measured on MI355X, bit-identical throughout: fp32
+// n=32, 65536 matrices: 301 us one-row, 172 us 8 x 4; a copy kernel with the
+// row-per-lane store pattern alone takes the same 172 us, with line-ordered
+// stores 142 us.  LOWER pays at fp32 n=64 (rows of two lines; 0.579 -> 0.530
+// ms at 65536) and not where a row is one line.  fp32 n=64 keeps the
+// row-per-lane stores: 16 KiB of staging would cost it half its waves.
 template <class T, int NB>
 struct PotrfShape {
-  static constexpr int L = NB, R = 1, WPB = waves_per_block<NB>();
+  static constexpr int L = NB, R = 1, WPB = waves_per_block<NB>(), OCC = 1;
+  static constexpr bool LOWER = false, STAGE = false;
 };
 template <>
 struct PotrfShape<float, 16> {
-  static constexpr int L = 4, R = 4, WPB = 1;
+  static constexpr int L = 4, R = 4, WPB = 1, OCC = 1;
+  static constexpr bool LOWER = false, STAGE = true;
 };
 template <>
 struct PotrfShape<float, 32> {
-  static constexpr int L = 8, R = 4, WPB = 1;
+  static constexpr int L = 8, R = 4, WPB = 1, OCC = 4;
+  static constexpr bool LOWER = false, STAGE = true;
 };
 template <>
 struct PotrfShape<float, 64> {
-  static constexpr int L = 32, R = 2, WPB = 1;
+  static constexpr int L = 32, R = 2, WPB = 1, OCC = 1;
+  static constexpr bool LOWER = true, STAGE = false;
 };
 template <>
 struct PotrfShape<double, 32> {
-  static constexpr int L = 8, R = 4, WPB = 1;
+  static constexpr int L = 8, R = 4, WPB = 1, OCC = 1;
+  static constexpr bool LOWER = false, STAGE = true;
 };
 
-template <class T, int L, int R, int WPB>
-__global__ __launch_bounds__(kWave * WPB) void k_potrf_rows(T* __restrict__ A, int n, int64_t batch,
-                                                           int* __restrict__ info, int vec) {
+// Buffer resource word 3 for gfx9 (raw 32-bit data format, no swizzle), and
+// an offset past the end of every resource built here (at most 2 MiB): a
+// buffer load at such an offset returns zeros without a memory access.
+constexpr int kBufferWord3 = 0x00020000;
+constexpr uint32_t kBufferOutOfRange = 0x7ffffff0u;
+
+// Loads row `row` of a lower Cholesky input, only as far as the diagonal:
+// the 16-byte chunks that hold an entry j <= row (the upper triangle is never
+// read by the factorization, and its output is zeros).  rs covers the wave's
+// matrices; off is the row's byte offset in it.  The chunks above the
+// diagonal, and every chunk of a row that is not loaded (ok false), take the
+// out-of-range offset, so the loads are issued without a branch and all of
+// them are in flight at once.  Entries not loaded are zero.  Needs rows of
+// whole 16-byte chunks (n * sizeof(T) a multiple of 16, 16-byte aligned A).
+template <class T, int N>
+__device__ __forceinline__ void load_row_lower(T (&r)[N], __amdgpu_buffer_rsrc_t rs, uint32_t off, int row, bool ok) {
+  constexpr int VE = Vec16<T>::N;
+  static_assert(N % VE == 0, "whole chunks");
+  typedef T vt __attribute__((ext_vector_type(VE)));
+  static_for<0, N / VE>([&](auto c) {
+    const bool in = ok && c * VE <= row;
+    const uint32_t o = in ? off + 16u * (uint32_t)c : kBufferOutOfRange;
+    const vt v = __builtin_bit_cast(vt, __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, 0));
+    static_for<0, VE>([&](auto e) { r[c * VE + e] = v[e()]; });
+  });
+}
+
+// Stores slot s (rows L*s .. L*s + L - 1) of every matrix of the wave with
+// whole-line writes.  A lane's own row (entries above the diagonal written as
+// zeros) spans L*(s+1) <= N columns of a 128-byte-multiple row; written by
+// one lane per row, every store instruction would touch 64 lines 16 bytes
+// at a time, and those partial-line writes, not the factorization, bound the
+// kernel.  So the rows go to LDS first (chunk c of row k at position
+// c ^ (k mod C), which keeps the 8 lanes of a 128-byte LDS window on distinct
+// banks both ways) and are read back in memory order: consecutive lanes then
+// write consecutive 16-byte chunks of the same rows.  Needs n == N (the
+// wave's matrices are N x N and contiguous) and 16-byte aligned rows.
+//   st: 64 * N elements of LDS (one slot of each of the wave's 64/L matrices).
+template <class T, int L, int N, int S>
+__device__ __forceinline__ void store_slot_lines(const T (&w)[N], int row, int g, int q, int lane, T* st, T* A,
+                                                 int64_t m0, int64_t batch) {
+  constexpr int VE = Vec16<T>::N, C = N / VE, MC = L * C;  // chunks per row, per matrix slot
+  static_assert((C & (C - 1)) == 0 && (kWave * C) % MC == 0, "power-of-two chunks per row");
+  typedef T vt __attribute__((ext_vector_type(VE)));
+  vt* sv = reinterpret_cast<vt*>(st);
+  static_for<0, C>([&](auto c) {
+    vt v;
+    static_for<0, VE>([&](auto e) {
+      constexpr int j = decltype(c)::value * VE + decltype(e)::value;
+      v[e()] = j > row ? T(0) : w[j];
+    });
+    sv[(g * L + q) * C + (c ^ (q & (C - 1)))] = v;
+  });
+  wave_sync();
+  static_for<0, C>([&](auto i) {
+    const int t = i * kWave + lane;
+    const int gg = t / MC, u = t % MC, k = u / C, j = u % C;
+    const vt v = sv[(gg * L + k) * C + (j ^ (k & (C - 1)))];
+    if (m0 + gg < batch)
+      *reinterpret_cast<vt*>(A + (m0 + gg) * (int64_t)(N * N) + (int64_t)(L * S + k) * N + j * VE) = v;
+  });
+}
+
+template <class T, int L, int R, int WPB, int OCC, bool LOWER, bool STAGE>
+__global__ __launch_bounds__(kWave * WPB) __attribute__((amdgpu_waves_per_eu(OCC))) void k_potrf_rows(
+    T* __restrict__ A, int n, int64_t batch, int* __restrict__ info, int vec) {
   constexpr int N = L * R;
   __shared__ T col_s[WPB][2 * kWave * R];
-  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  __shared__ __align__(16) T stage_s[WPB][STAGE ? kWave * N : 4];
+  const int lane = threadIdx.x % kWave, wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int g = lane / L, q = lane % L;
-  const int64_t m = ((int64_t)blockIdx.x * WPB + wv) * (kWave / L) + g;
+  const int64_t m0 = ((int64_t)blockIdx.x * WPB + wv) * (kWave / L);  // the wave's first matrix
+  const int64_t m = m0 + g;
   const bool live = m < batch;
   const int64_t nn = (int64_t)n * n;
   T r[R][N];
-  static_for<0, R>([&](auto s) {
-    const int row = q + L * s;
-    load_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, live && row < n, vec != 0);
-  });
+  if (LOWER && vec) {
+    // one resource over the wave's matrices (at most 64 / L of them)
+    const int64_t left = batch - m0;
+    const int cnt = left <= 0 ? 0 : (left < kWave / L ? (int)left : kWave / L);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        A + (left <= 0 ? 0 : m0 * nn), (short)0, (int)(cnt * nn * (int64_t)sizeof(T)), kBufferWord3);
+    static_for<0, R>([&](auto s) {
+      const int row = q + L * s;
+      const uint32_t off = (uint32_t)(((int64_t)g * nn + (int64_t)row * n) * (int64_t)sizeof(T));
+      load_row_lower<T, N>(r[s], rs, off, row, live && row < n);
+    });
+  } else {
+    static_for<0, R>([&](auto s) {
+      const int row = q + L * s;
+      load_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, live && row < n, vec != 0);
+    });
+  }
   // each slot is stored as soon as its rows are final: the writes of the
   // first R - 1 slots overlap the rest of the factorization
   auto store_slot = [&](auto sc) {
     constexpr int s = decltype(sc)::value;
     const int row = q + L * s;
-    if (live && row < n) store_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, vec != 0, true, row);
+    T w[N];  // the columns right of the slot's rows are zeros by position
+    static_for<0, N>([&](auto j) { w[j] = j < L * (s + 1) ? r[s][j] : T(0); });
+    if (STAGE && vec && n == N) {
+      store_slot_lines<T, L, N, s>(w, row, g, q, lane, stage_s[wv], A, m0, batch);
+      return;
+    }
+    if (live && row < n) store_row<T, N>(w, A + m * nn + (int64_t)row * n, n, vec != 0, true, row);
   };
   const int inf = chol_rows<T, L, R>(r, n, n, q, g, col_s[wv], store_slot);
   store_slot(ic<R - 1>{});
