@@ -184,14 +184,51 @@ def test_getrf_multirow_batches_bit_exact(hip, oracle, n):
     assert bits_equal(dG.cpu().numpy(), LUo)
 
 
+@pytest.mark.parametrize("dt,n,batch", [(np.float32, 8, 4099), (np.float32, 16, 16387), (np.float32, 32, 65537),
+                                         (np.float32, 64, 1029), (np.float64, 8, 2051), (np.float64, 16, 4097),
+                                         (np.float64, 32, 32771), (np.float64, 64, 8195)])
+def test_getrf_full_bucket_batches_bit_exact(hip, oracle, dt, n, batch):
+    """n equal to its bucket (8, 16, 32, 64): the one-row LU kernel writes the
+    factor through LDS in line order; factors, pivots and info bit-exact,
+    zero columns, a zero matrix, NaN, inf and ties included, and a batch that
+    leaves the last wave partly empty between two guard regions that must
+    come back untouched."""
+    import torch
+    from tinysolverdx import device
+    rng = np.random.default_rng(3000 + n)
+    G = rng.standard_normal((batch, n, n)).astype(dt)
+    zc = rng.choice(batch, batch // 100, replace=False)
+    G[zc, :, rng.integers(0, n, batch // 100)] = 0.0
+    G[7] = 0.0
+    G[11, 5] = G[11, 2]
+    G[13, 3, 4] = np.nan
+    G[17, :, 0] = 1.0
+    G[19, n - 1, n - 1] = np.inf
+    print(f"[getrf full bucket] {np.dtype(dt).name} n={n} batch={batch}", flush=True)
+    tdt = torch.float32 if dt == np.float32 else torch.float64
+    buf = torch.full((batch + 2, n, n), 7.0, dtype=tdt, device="cuda")
+    dG = buf[1:batch + 1]
+    dG.copy_(torch.from_numpy(G))
+    piv = torch.zeros((batch, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    device.getrf(dG, piv, info)
+    LUo, pivo, io = oracle.lu(G)
+    torch.cuda.synchronize()
+    assert np.array_equal(info.cpu().numpy(), io)
+    assert np.array_equal(piv.cpu().numpy(), pivo)
+    assert bits_equal(dG.cpu().numpy(), LUo)
+    assert bool((buf[0] == 7.0).all()) and bool((buf[batch + 1] == 7.0).all())
+
+
 @pytest.mark.parametrize("dt,n,batch", [(np.float32, 32, 65536), (np.float32, 13, 131072), (np.float32, 50, 16384),
                                          (np.float64, 29, 32768), (np.float32, 32, 65531), (np.float32, 16, 65539),
-                                         (np.float64, 32, 32771), (np.float32, 64, 16387)])
+                                         (np.float64, 32, 32771), (np.float32, 64, 16387), (np.float32, 8, 8195),
+                                         (np.float64, 8, 4097), (np.float64, 16, 4099)])
 def test_potrf_persistent_batches_bit_exact(hip, oracle, dt, n, batch):
     """Batches of several rounds of waves of the multi-row Cholesky
     (k_potrf_rows; fp32 n=32 is config 0): factors and info bit-exact against
     the oracle, failing matrices (tile semantics) included.  The full-bucket
-    sizes (16, 32, 64) take the line-ordered stores through LDS or the
+    sizes (8, 16, 32, 64) take the line-ordered stores through LDS or the
     lower-triangle loads; batches that are not a multiple of the matrices per
     wave leave the last wave partly empty, whose absent matrices must not be
     written."""
