@@ -314,10 +314,16 @@ def _line(metric, value, unit, a, ms, dtype, config, roofline, cpu):
             "cpu_baseline": cpu}
 
 
-def _roof(bytes_total, sec, kernel, per_unit):
+def _roof(bytes_total, sec, kernel, per_unit, traffic_key=None):
     ach = bytes_total / sec / 1e9
+    traffic = None
+    try:  # HBM bytes per launch measured by rocprofv3 for this workload (profiles/traffic.json)
+        if traffic_key:
+            traffic = json.loads((ROOT / "profiles" / "traffic.json").read_text())[traffic_key][kernel]
+    except (OSError, KeyError, ValueError):
+        traffic = None
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel, "bytes_per_matrix": per_unit}
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel, "bytes_per_matrix": per_unit}
 
 
 def _cpu_timed(fn, units, unit, sample):
@@ -384,7 +390,8 @@ def run_potrf(a):
                                      "[1e-3,1]", "info_gt0": bad, "max_rel_residual_sampled": res,
                          "gflops": round(batch / (ms * 1e-3) * n ** 3 / 3 / 1e9, 1),
                          "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)},
-                 _roof(per * batch, ms * 1e-3, "k_potrf_rows<float,8,4,1,4,false,true>", per), cpu)
+                 _roof(per * batch, ms * 1e-3, "k_potrf_rows<float,8,4,1,4,false,true>", per,
+                       f"potrf_f32_n{n}_b{batch}"), cpu)
 
 
 def run_getrf(a):
@@ -430,7 +437,8 @@ def run_getrf(a):
            "gesv4_GBps": round(per_s * batch / (ms_s * 1e-3) / 1e9, 1),
            "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)}
     return _line(f"getrf matrices/s {a.dtype} n={n} (~1 GiB of A); HBM GB/s", batch / (ms * 1e-3), "matrices/s",
-                 a, ms, a.dtype, cfg, _roof(per * batch, ms * 1e-3, _getrf_kernel(es, n, batch), per), cpu)
+                 a, ms, a.dtype, cfg, _roof(per * batch, ms * 1e-3, _getrf_kernel(es, n, batch), per,
+                       f"getrf_{a.dtype}_n{n}_b{batch}"), cpu)
 
 
 def _getrf_kernel(es, n, batch):
