@@ -21,11 +21,9 @@ void check_count(int64_t count) {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-// routed workspace: [fallback count (256 B)] [fallback list]
-size_t routed_workspace(int64_t batch) {
-  const size_t b = batch < 0 ? 0 : (size_t)batch;
-  return 256 + align256(b * sizeof(int));
-}
+// routed workspace: [fallback count (256 B)] [fallback list] [regrouped weights]
+size_t routed_list_bytes(int64_t batch) { return align256((batch < 0 ? 0 : (size_t)batch) * sizeof(int)); }
+size_t routed_workspace(int64_t batch) { return 256 + routed_list_bytes(batch) + align256(kRouterTileBytes); }
 
 }  // namespace
 
@@ -106,7 +104,8 @@ int tsdx_hip_posv_routed_f64(const double* A, int n, int64_t batch, const double
     a.stat = stat;
     a.flist = flist;
     a.fcount = fcount;
-    hip_check(launch_posv_routed(a, params, tier, feat, S(stream)), "routed posv launch");
+    float* wt = reinterpret_cast<float*>(w + 256 + routed_list_bytes(batch));
+    hip_check(launch_posv_routed(a, params, wt, tier, feat, S(stream)), "routed posv launch");
   });
 }
 

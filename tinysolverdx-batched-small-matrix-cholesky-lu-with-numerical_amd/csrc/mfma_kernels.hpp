@@ -645,21 +645,37 @@ __device__ __forceinline__ void mx_features(const double (&ad)[nupper<NBLK>()][4
   f[11] = ddom * rn;
 }
 
+// W1 and W2 regrouped for the network below: element (o, k) of W (row o =
+// hidden unit o) at ((k / 4) * 64 + o) * 4 + k % 4, so that lane o's four
+// weights k .. k+3 are one 16-byte chunk and the 64 lanes read 1 KiB of
+// consecutive chunks (in the parameter layout each lane reads its own row,
+// and every load touches 64 lines).  W1t: 3 chunks per lane, then W2t: 16
+// (kTileW1, kTileW2, kTileFloats in router_layout.hpp; written by
+// k_router_tiles, router.hip).
+
 // The network on one wave (lane o = hidden unit o); returns the tier, the
 // first largest logit (a NaN logit gives tier 2).  hs: 64 floats of LDS.
-// The weights are read where they are used (row o of W1 and W2, column o of
-// W3), which keeps them out of the registers that hold the matrix.
-__device__ __forceinline__ int mx_route(const float* __restrict__ P, const float (&f)[kRF], int lane, float* hs) {
+// The weights are read where they are used (W1 and W2 from the regrouped copy
+// Wt, column o of W3 from P), which keeps them out of the registers that hold
+// the matrix.  The sums run in the same order as k_router_forward's.
+__device__ __forceinline__ int mx_route(const float* __restrict__ P, const float* __restrict__ Wt,
+                                        const float (&f)[kRF], int lane, float* hs) {
   float a1 = P[kOffB1 + lane];
-  static_for<0, kRF>([&](auto q) {
-    a1 = __builtin_fmaf(P[kOffW1 + lane * kRF + q], (f[q] - P[q]) * __builtin_amdgcn_rcpf(P[kRF + q]), a1);
+  const float4* w1 = reinterpret_cast<const float4*>(Wt + kTileW1) + lane;
+  static_for<0, kRF / 4>([&](auto c) {
+    const float4 w = w1[c * kRH];
+    const float wq[4] = {w.x, w.y, w.z, w.w};
+    static_for<0, 4>([&](auto e) {
+      constexpr int q = 4 * decltype(c)::value + decltype(e)::value;
+      a1 = __builtin_fmaf(wq[e], (f[q] - P[q]) * __builtin_amdgcn_rcpf(P[kRF + q]), a1);
+    });
   });
   hs[lane] = a1 > 0.0f ? a1 : 0.0f;
   wave_sync();
   float a2 = P[kOffB2 + lane];
-  const float4* w2 = reinterpret_cast<const float4*>(P + kOffW2 + lane * kRH);  // kOffW2 is a multiple of 4
+  const float4* w2 = reinterpret_cast<const float4*>(Wt + kTileW2) + lane;
   static_for<0, kRH / 4>([&](auto q) {
-    const float4 w = w2[q];
+    const float4 w = w2[q * kRH];
     const float4 h = *reinterpret_cast<const float4*>(hs + 4 * q);
     a2 = __builtin_fmaf(w.x, h.x, a2);
     a2 = __builtin_fmaf(w.y, h.y, a2);
@@ -684,7 +700,8 @@ __device__ __forceinline__ int mx_route(const float* __restrict__ P, const float
 // (optional) receive the choice and the features.
 template <int NBLK>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void k_posv_mx_routed(
-    IrArgs a, const float* __restrict__ P, int* __restrict__ tier_out, float* __restrict__ feat_out) {
+    IrArgs a, const float* __restrict__ P, const float* __restrict__ Wt, int* __restrict__ tier_out,
+    float* __restrict__ feat_out) {
   __shared__ __align__(16) double xs[16 * NBLK];
   __shared__ __align__(16) float fs[16];
   __shared__ __align__(16) float hs[kRH];
@@ -701,7 +718,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     static_for<1, kRF>([&](auto q) { v = (lane == q) ? f[q] : v; });
     feat_out[m * kRF + lane] = v;
   }
-  const int t = mx_route(P, f, lane, hs);
+  const int t = mx_route(P, Wt, f, lane, hs);
   if (tier_out && lane == 0) tier_out[m] = t;
   if (t == 2) {
     if (lane == 0) {

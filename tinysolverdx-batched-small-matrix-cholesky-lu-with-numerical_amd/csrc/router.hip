@@ -41,22 +41,34 @@ hipError_t launch_router_adam(float* P, const float* grad, float* m, float* v, i
   return hipGetLastError();
 }
 
+// W1 and W2 of the parameter block P regrouped as the in-wave network reads
+// them (mx_route): element (o, k) at ((k / 4) * 64 + o) * 4 + k % 4.
+__global__ void k_router_tiles(const float* __restrict__ P, float* __restrict__ W) {
+  for (int e = threadIdx.x; e < kTileFloats; e += blockDim.x) {
+    const bool w1 = e < kTileW2;
+    const int x = w1 ? e : e - kTileW2, K = w1 ? kRF : kRH;
+    const int c = x / (kRH * 4), o = (x / 4) % kRH, k = 4 * c + x % 4;
+    W[e] = P[(w1 ? kOffW1 : kOffW2) + o * K + k];
+  }
+}
+
 __global__ void k_zero_counts(int* p, int k) {
   if ((int)threadIdx.x < k) p[threadIdx.x] = 0;
 }
 
-hipError_t launch_posv_routed(const IrArgs& a, const float* P, int* tier, float* feat, hipStream_t s) {
+hipError_t launch_posv_routed(const IrArgs& a, const float* P, float* Wt, int* tier, float* feat, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, a.fcount, 1);
+  hipLaunchKernelGGL(k_router_tiles, dim3(1), dim3(256), 0, s, P, Wt);
   const dim3 grid((unsigned)a.batch), block(kWave);
   const int nblk = (a.n + 15) / 16;
   if (nblk == 1)
-    hipLaunchKernelGGL((k_posv_mx_routed<1>), grid, block, 0, s, a, P, tier, feat);
+    hipLaunchKernelGGL((k_posv_mx_routed<1>), grid, block, 0, s, a, P, Wt, tier, feat);
   else if (nblk == 2)
-    hipLaunchKernelGGL((k_posv_mx_routed<2>), grid, block, 0, s, a, P, tier, feat);
+    hipLaunchKernelGGL((k_posv_mx_routed<2>), grid, block, 0, s, a, P, Wt, tier, feat);
   else if (nblk == 3)
-    hipLaunchKernelGGL((k_posv_mx_routed<3>), grid, block, 0, s, a, P, tier, feat);
+    hipLaunchKernelGGL((k_posv_mx_routed<3>), grid, block, 0, s, a, P, Wt, tier, feat);
   else
-    hipLaunchKernelGGL((k_posv_mx_routed<4>), grid, block, 0, s, a, P, tier, feat);
+    hipLaunchKernelGGL((k_posv_mx_routed<4>), grid, block, 0, s, a, P, Wt, tier, feat);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return launch_posv_gather(a, s);

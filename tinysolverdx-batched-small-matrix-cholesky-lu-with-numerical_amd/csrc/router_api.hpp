@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "mixed_api.hpp"
+#include "router_layout.hpp"
 
 namespace tsdx_hip {
 
@@ -19,6 +20,8 @@ hipError_t launch_router_adam(float* P, const float* grad, float* m, float* v, i
 // network and solves in the chosen tier; then the float64 solve of tier-2
 // matrices and of every fallback.  tier / feat (optional) receive the choice
 // and the features.
-hipError_t launch_posv_routed(const IrArgs& a, const float* P, int* tier, float* feat, hipStream_t s);
+// Wt: kTileFloats floats of device memory (16-byte aligned) for the regrouped
+// first and second layer weights.
+hipError_t launch_posv_routed(const IrArgs& a, const float* P, float* Wt, int* tier, float* feat, hipStream_t s);
 
 }  // namespace tsdx_hip

@@ -6,12 +6,17 @@
 // 1 = fp32 + refinement, 2 = float64 direct.
 #pragma once
 
+#include <stddef.h>
+
 namespace tsdx_hip {
 
 constexpr int kRF = 12, kRH = 64, kRT = 3;
 constexpr int kOffW1 = 24, kOffB1 = kOffW1 + kRH * kRF, kOffW2 = kOffB1 + kRH, kOffB2 = kOffW2 + kRH * kRH,
               kOffW3 = kOffB2 + kRH, kOffB3 = kOffW3 + kRT * kRH, kRouterParams = kOffB3 + kRT;
 static_assert(kRouterParams == 5211, "parameter layout");
+// The regrouped W1 and W2 the in-wave network reads (k_router_tiles).
+constexpr int kTileW1 = 0, kTileW2 = kRH * kRF, kTileFloats = kTileW2 + kRH * kRH;
+constexpr size_t kRouterTileBytes = kTileFloats * sizeof(float);
 
 // log10 with non-positive (and tiny) arguments clamped to -30: log2 on the
 // transcendental unit (arguments past the clamp are normal numbers, which it
