@@ -390,7 +390,7 @@ def run_potrf(a):
                                      "[1e-3,1]", "info_gt0": bad, "max_rel_residual_sampled": res,
                          "gflops": round(batch / (ms * 1e-3) * n ** 3 / 3 / 1e9, 1),
                          "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)},
-                 _roof(per * batch, ms * 1e-3, "k_potrf_rows<float,8,4,1,4,false,true>", per,
+                 _roof(per * batch, ms * 1e-3, "k_potrf_rows<float,8,4,1,1,false,true>", per,
                        f"potrf_f32_n{n}_b{batch}"), cpu)
 
 

@@ -204,9 +204,12 @@ __device__ __forceinline__ void lu_rows(T (&r)[R][L * R], int nloop, int n, int 
 // (store_slot_lines).  Measured on MI355X, bit-identical throughout: fp32
 // n=32, 65536 matrices: 301 us one-row, 172 us 8 x 4; a copy kernel with the
 // row-per-lane store pattern alone takes the same 172 us, with line-ordered
-// stores 142 us.  LOWER pays at fp32 n=64 (rows of two lines; 0.579 -> 0.530
-// ms at 65536) and not where a row is one line.  fp32 n=64 keeps the
-// row-per-lane stores: 16 KiB of staging would cost it half its waves.
+// stores 142 us; the kernel with STAGE 136 us.  OCC 4 (128 VGPRs, possible
+// once slots keep only the columns their rows reach) measured 1 % slower than
+// the default 3 waves per SIMD there, so OCC stays 1.  LOWER pays at fp32 n=64
+// (rows of two lines; 0.580 -> 0.572 ms at 65536) and not where a row is one
+// line.  fp32 n=64 keeps the row-per-lane stores: 16 KiB of staging would
+// cost it half its waves.
 template <class T, int NB>
 struct PotrfShape {
   static constexpr int L = NB, R = 1, WPB = waves_per_block<NB>(), OCC = 1;
@@ -219,7 +222,7 @@ struct PotrfShape<float, 16> {
 };
 template <>
 struct PotrfShape<float, 32> {
-  static constexpr int L = 8, R = 4, WPB = 1, OCC = 4;
+  static constexpr int L = 8, R = 4, WPB = 1, OCC = 1;
   static constexpr bool LOWER = false, STAGE = true;
 };
 template <>
