@@ -190,8 +190,19 @@ struct Vec16<double> {
   static constexpr int N = 2;
 };
 
+// A 16-byte vector of T that needs only T's alignment: global loads and
+// stores of whole 16-byte chunks at any element offset (the gfx9 memory path
+// takes them; the compiler itself emits such accesses for element-aligned
+// vectors).
+template <class T>
+using VecU = T __attribute__((ext_vector_type(Vec16<T>::N), aligned(sizeof(T))));
+
 // Loads r[0..n-1] from p (contiguous).  vec: p is 16-byte aligned and n*sizeof(T)
-// is a multiple of 16, so 128-bit loads are used.  Other entries are zero.
+// is a multiple of 16, so aligned 128-bit loads are used.  Otherwise the row is
+// read as 16-byte chunks at its own element offset, the last partial chunk
+// element by element (a row read one element per instruction makes each
+// load touch as many lines as the wave has rows, and costs 2-2.5x at n=49 or
+// 57).  Other entries are zero.
 template <class T, int NB>
 __device__ __forceinline__ void load_row(T (&r)[NB], const T* __restrict__ p, int n, bool ok, bool vec) {
   static_for<0, NB>([&](auto j) { r[j] = T(0); });
@@ -212,13 +223,25 @@ __device__ __forceinline__ void load_row(T (&r)[NB], const T* __restrict__ p, in
       });
       return;
     }
+    static_for<0, NB / VE>([&](auto q) {
+      if (q * VE + VE <= n) {  // n is uniform: no divergence
+        const VecU<T> v = *reinterpret_cast<const VecU<T>*>(p + q * VE);
+        static_for<0, VE>([&](auto e) { r[q * VE + e] = v[e()]; });
+      } else {
+        static_for<0, VE>([&](auto e) {
+          if (q * VE + e < n) r[q * VE + e] = p[q * VE + e];
+        });
+      }
+    });
+    return;
   }
   static_for<0, NB>([&](auto j) {
     if (j < n) r[j] = p[j];
   });
 }
 
-// Stores r[0..n-1] to p; with lower_only, entries j > i are written as zero.
+// Stores r[0..n-1] to p (chunked like load_row); with lower_only, entries
+// j > i are written as zero.
 template <class T, int NB>
 __device__ __forceinline__ void store_row(const T (&r)[NB], T* __restrict__ p, int n, bool vec, bool lower_only, int i) {
   constexpr int VE = Vec16<T>::N;
@@ -240,6 +263,20 @@ __device__ __forceinline__ void store_row(const T (&r)[NB], T* __restrict__ p, i
       });
       return;
     }
+  }
+  if constexpr (NB % VE == 0) {  // whole chunks at the row's own offset, as load_row
+    static_for<0, NB / VE>([&](auto q) {
+      if (q * VE + VE <= n) {
+        VecU<T> v;
+        static_for<0, VE>([&](auto e) { v[e()] = val(q * VE + e, r[q * VE + e]); });
+        *reinterpret_cast<VecU<T>*>(p + q * VE) = v;
+      } else {
+        static_for<0, VE>([&](auto e) {
+          if (q * VE + e < n) p[q * VE + e] = val(q * VE + e, r[q * VE + e]);
+        });
+      }
+    });
+    return;
   }
   static_for<0, NB>([&](auto j) {
     if (j < n) p[j] = val(j, r[j]);
