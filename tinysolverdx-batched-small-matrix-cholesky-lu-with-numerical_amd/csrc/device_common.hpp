@@ -197,18 +197,20 @@ struct Vec16<double> {
 template <class T>
 using VecU = T __attribute__((ext_vector_type(Vec16<T>::N), aligned(sizeof(T))));
 
-// Loads r[0..n-1] from p (contiguous).  vec: p is 16-byte aligned and n*sizeof(T)
-// is a multiple of 16, so aligned 128-bit loads are used.  Otherwise the row is
-// read as 16-byte chunks at its own element offset, the last partial chunk
-// element by element (a row read one element per instruction makes each
-// load touch as many lines as the wave has rows, and costs 2-2.5x at n=49 or
-// 57).  Other entries are zero.
-template <class T, int NB>
+// Loads r[0..n-1] from p (contiguous); other entries are zero.  CHUNKED: as
+// 16-byte chunks at the row's own element offset, the partial last chunk
+// element by element, whatever the alignment (a row read one element per
+// instruction makes each load touch as many lines as the wave has rows,
+// which cost 2-2.5x at n = 49 or 57).  Otherwise aligned chunks when vec (p
+// 16-byte aligned, the row whole chunks), else element by element: the posv
+// and gesv kernels keep that form, which the compiler fits in fewer registers
+// there (gesv fp32 n=32: 61 against 89 VGPRs, posv 81 against 87).
+template <class T, int NB, bool CHUNKED = true>
 __device__ __forceinline__ void load_row(T (&r)[NB], const T* __restrict__ p, int n, bool ok, bool vec) {
   static_for<0, NB>([&](auto j) { r[j] = T(0); });
   if (!ok) return;
   constexpr int VE = Vec16<T>::N;
-  if constexpr (NB % VE == 0) {
+  if constexpr (!CHUNKED && NB % VE == 0) {
     if (vec) {
       using V = typename Vec16<T>::type;
       static_for<0, NB / VE>([&](auto q) {
@@ -223,16 +225,20 @@ __device__ __forceinline__ void load_row(T (&r)[NB], const T* __restrict__ p, in
       });
       return;
     }
+  }
+  if constexpr (CHUNKED && NB % VE == 0) {
     static_for<0, NB / VE>([&](auto q) {
       if (q * VE + VE <= n) {  // n is uniform: no divergence
         const VecU<T> v = *reinterpret_cast<const VecU<T>*>(p + q * VE);
         static_for<0, VE>([&](auto e) { r[q * VE + e] = v[e()]; });
-      } else {
-        static_for<0, VE>([&](auto e) {
-          if (q * VE + e < n) r[q * VE + e] = p[q * VE + e];
-        });
       }
     });
+    if (n % VE != 0) {  // the partial last chunk, element by element
+      const int q0 = n - n % VE;
+      static_for<0, NB>([&](auto j) {
+        if (j >= q0 && j < n) r[j] = p[j];
+      });
+    }
     return;
   }
   static_for<0, NB>([&](auto j) {
@@ -240,13 +246,13 @@ __device__ __forceinline__ void load_row(T (&r)[NB], const T* __restrict__ p, in
   });
 }
 
-// Stores r[0..n-1] to p (chunked like load_row); with lower_only, entries
+// Stores r[0..n-1] to p, chunked like load_row; with lower_only, entries
 // j > i are written as zero.
-template <class T, int NB>
+template <class T, int NB, bool CHUNKED = true>
 __device__ __forceinline__ void store_row(const T (&r)[NB], T* __restrict__ p, int n, bool vec, bool lower_only, int i) {
   constexpr int VE = Vec16<T>::N;
   auto val = [&](int j, T x) { return (lower_only && j > i) ? T(0) : x; };
-  if constexpr (NB % VE == 0) {
+  if constexpr (!CHUNKED && NB % VE == 0) {
     if (vec) {
       using V = typename Vec16<T>::type;
       static_for<0, NB / VE>([&](auto q) {
@@ -264,18 +270,20 @@ __device__ __forceinline__ void store_row(const T (&r)[NB], T* __restrict__ p, i
       return;
     }
   }
-  if constexpr (NB % VE == 0) {  // whole chunks at the row's own offset, as load_row
+  if constexpr (CHUNKED && NB % VE == 0) {
     static_for<0, NB / VE>([&](auto q) {
       if (q * VE + VE <= n) {
         VecU<T> v;
         static_for<0, VE>([&](auto e) { v[e()] = val(q * VE + e, r[q * VE + e]); });
         *reinterpret_cast<VecU<T>*>(p + q * VE) = v;
-      } else {
-        static_for<0, VE>([&](auto e) {
-          if (q * VE + e < n) p[q * VE + e] = val(q * VE + e, r[q * VE + e]);
-        });
       }
     });
+    if (n % VE != 0) {
+      const int q0 = n - n % VE;
+      static_for<0, NB>([&](auto j) {
+        if (j >= q0 && j < n) p[j] = val(j, r[j]);
+      });
+    }
     return;
   }
   static_for<0, NB>([&](auto j) {

@@ -34,7 +34,7 @@ __device__ __forceinline__ void posv_plain_group(const IrArgs& a, int64_t m, boo
   const int64_t nn = (int64_t)n * n;
   const bool io = live && i < n;
   double r[NB];
-  load_row<double, NB>(r, a.A + m * nn + (int64_t)i * n, n, io, false);
+  load_row<double, NB, false>(r, a.A + m * nn + (int64_t)i * n, n, io, false);
   const int inf = chol_regs<double, NB>(r, n, n, i, g, col_s);
   if (live && i == 0 && a.info) a.info[m] = inf;
   const double dinv = diag_inv<double, NB>(r, i);

@@ -436,9 +436,9 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_posv(const T*
   const int64_t nn = (int64_t)n * n;
   const bool io = t.live && t.i < n;
   T r[NB];
-  load_row<T, NB>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
+  load_row<T, NB, false>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
   const int inf = chol_regs<T, NB>(r, n, n, t.i, t.g, col_s[wv]);
-  if (Lout && io) store_row<T, NB>(r, Lout + t.m * nn + (int64_t)t.i * n, n, vec != 0, true, t.i);
+  if (Lout && io) store_row<T, NB, false>(r, Lout + t.m * nn + (int64_t)t.i * n, n, vec != 0, true, t.i);
   if (t.live && t.i == 0 && info) info[t.m] = inf;
   const T dinv = diag_inv<T, NB>(r, t.i);
   for (int c0 = 0; c0 < nrhs; c0 += NR) {
@@ -463,11 +463,11 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_gesv(const T*
   const int64_t nn = (int64_t)n * n;
   const bool io = t.live && t.i < n;
   T r[NB];
-  load_row<T, NB>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
+  load_row<T, NB, false>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
   int pos, mypiv, inf;
   lu_regs<T, NB, ZTILE>(r, n, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
   if (io) {
-    if (LUout) store_row<T, NB>(r, LUout + t.m * nn + (int64_t)pos * n, n, vec != 0, false, 0);
+    if (LUout) store_row<T, NB, false>(r, LUout + t.m * nn + (int64_t)pos * n, n, vec != 0, false, 0);
     if (piv) piv[t.m * n + t.i] = mypiv;
   }
   if (t.live && t.i == 0 && info) info[t.m] = inf;
