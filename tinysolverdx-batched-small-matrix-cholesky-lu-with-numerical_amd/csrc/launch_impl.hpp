@@ -26,7 +26,7 @@ hipError_t launch_potrf(T* A, int n, int64_t batch, int* info, int vec, hipStrea
   } else {
     constexpr int64_t per_block = (int64_t)S::WPB * (kWave / S::L);
     const dim3 grid((unsigned)((batch + per_block - 1) / per_block));
-    hipLaunchKernelGGL((k_potrf_rows<T, S::L, S::R, S::WPB, S::OCC, S::LOWER, S::STAGE>), grid, dim3(kWave * S::WPB), 0, s, A, n, batch, info, vec);
+    hipLaunchKernelGGL((k_potrf_rows<T, S::L, S::R, S::WPB, S::OCC, S::LOWER, S::STAGE, S::CHUNK>), grid, dim3(kWave * S::WPB), 0, s, A, n, batch, info, vec);
   }
   return hipGetLastError();
 }

@@ -209,31 +209,34 @@ __device__ __forceinline__ void lu_rows(T (&r)[R][L * R], int nloop, int n, int 
 // the default 3 waves per SIMD there, so OCC stays 1.  LOWER pays at fp32 n=64
 // (rows of two lines; 0.580 -> 0.572 ms at 65536) and not where a row is one
 // line.  fp32 n=64 keeps the row-per-lane stores: 16 KiB of staging would
-// cost it half its waves.
+// cost it half its waves.  CHUNK: rows that are not whole 16-byte chunks
+// load and store as chunks (load_row, store_row); fp32 n=32 keeps the
+// aligned-or-element form, whose 136 VGPRs hold it at the 3 waves per SIMD
+// measured fastest (chunked, 122 VGPRs and 4 waves: 0.1385 against 0.1339 ms).
 template <class T, int NB>
 struct PotrfShape {
   static constexpr int L = NB, R = 1, WPB = waves_per_block<NB>(), OCC = 1;
-  static constexpr bool LOWER = false, STAGE = false;
+  static constexpr bool LOWER = false, STAGE = false, CHUNK = true;
 };
 template <>
 struct PotrfShape<float, 16> {
   static constexpr int L = 4, R = 4, WPB = 1, OCC = 1;
-  static constexpr bool LOWER = false, STAGE = true;
+  static constexpr bool LOWER = false, STAGE = true, CHUNK = true;
 };
 template <>
 struct PotrfShape<float, 32> {
   static constexpr int L = 8, R = 4, WPB = 1, OCC = 1;
-  static constexpr bool LOWER = false, STAGE = true;
+  static constexpr bool LOWER = false, STAGE = true, CHUNK = false;
 };
 template <>
 struct PotrfShape<float, 64> {
   static constexpr int L = 32, R = 2, WPB = 1, OCC = 1;
-  static constexpr bool LOWER = true, STAGE = false;
+  static constexpr bool LOWER = true, STAGE = false, CHUNK = true;
 };
 template <>
 struct PotrfShape<double, 32> {
   static constexpr int L = 8, R = 4, WPB = 1, OCC = 1;
-  static constexpr bool LOWER = false, STAGE = true;
+  static constexpr bool LOWER = false, STAGE = true, CHUNK = true;
 };
 
 // Buffer resource word 3 for gfx9 (raw 32-bit data format, no swizzle), and
@@ -299,7 +302,7 @@ __device__ __forceinline__ void store_slot_lines(const T (&w)[N], int row, int g
   });
 }
 
-template <class T, int L, int R, int WPB, int OCC, bool LOWER, bool STAGE>
+template <class T, int L, int R, int WPB, int OCC, bool LOWER, bool STAGE, bool CHUNK>
 __global__ __launch_bounds__(kWave * WPB) __attribute__((amdgpu_waves_per_eu(OCC))) void k_potrf_rows(
     T* __restrict__ A, int n, int64_t batch, int* __restrict__ info, int vec) {
   constexpr int N = L * R;
@@ -326,7 +329,7 @@ __global__ __launch_bounds__(kWave * WPB) __attribute__((amdgpu_waves_per_eu(OCC
   } else {
     static_for<0, R>([&](auto s) {
       const int row = q + L * s;
-      load_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, live && row < n, vec != 0);
+      load_row<T, N, CHUNK>(r[s], A + m * nn + (int64_t)row * n, n, live && row < n, vec != 0);
     });
   }
   // each slot is stored as soon as its rows are final: the writes of the
@@ -340,7 +343,7 @@ __global__ __launch_bounds__(kWave * WPB) __attribute__((amdgpu_waves_per_eu(OCC
       store_slot_lines<T, L, N, s>(w, row, g, q, lane, stage_s[wv], A, m0, batch);
       return;
     }
-    if (live && row < n) store_row<T, N>(w, A + m * nn + (int64_t)row * n, n, vec != 0, true, row);
+    if (live && row < n) store_row<T, N, CHUNK>(w, A + m * nn + (int64_t)row * n, n, vec != 0, true, row);
   };
   const int inf = chol_rows<T, L, R>(r, n, n, q, g, col_s[wv], store_slot);
   store_slot(ic<R - 1>{});
