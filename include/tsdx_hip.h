@@ -158,7 +158,10 @@ int tsdx_hip_router_adam(float *params, const float *grad, float *m, float *v, i
  * solve.  stat as for tsdx_hip_posv_ir_f64; tier (batch) and feat (batch, 12)
  * may be NULL, else they receive the network's choice and the features (the
  * same definitions as tsdx_hip_router_features_f64).  Only the lower triangle
- * of A is read. */
+ * of A is read.  work: device workspace of
+ * tsdx_hip_posv_routed_workspace_size(batch) bytes (the fallback list and a
+ * per-call copy of the first two weight layers); calls sharing one workspace
+ * must not overlap. */
 size_t tsdx_hip_posv_routed_workspace_size(int64_t batch);
 int tsdx_hip_posv_routed_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs,
                              const float *params, int max_iter, double tol, int *info, int *stat, int *tier,
