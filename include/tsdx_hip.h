@@ -183,8 +183,12 @@ int tsdx_hip_posv_routed_f64(const double *A, int n, int64_t batch, const double
  * of each size class on synthetic matrices of the same sizes (the caller's
  * data is not touched) and keeps the fastest.  It captures the work - one
  * persistent kernel per size class fed by a work queue - into a HIP graph;
- * tsdx_hip_ragged_run replays it on `stream` (one graph launch).  The plan is
- * bound to the buffers it was created with; runs of one plan must not overlap.
+ * tsdx_hip_ragged_run replays it on `stream` (one graph launch).  The plan
+ * keeps its own copy of sizes and kinds (later changes to the caller's arrays
+ * do not reach it) and is bound to A, offsets, piv, piv_offsets and info,
+ * whose addresses are captured: offsets and piv_offsets must not change after
+ * planning.  Runs of one plan must not overlap (on one stream or several):
+ * they share the plan's work counters.
  * ------------------------------------------------------------------------ */
 #define TSDX_HIP_KIND_CHOLESKY 0
 #define TSDX_HIP_KIND_LU 1
@@ -202,6 +206,32 @@ int tsdx_hip_ragged_run(tsdx_hip_ragged_plan *plan, void *stream);
  * the tuner's timings. */
 const char *tsdx_hip_ragged_plan_report(const tsdx_hip_ragged_plan *plan);
 int tsdx_hip_ragged_plan_destroy(tsdx_hip_ragged_plan *plan);
+
+/* ---------------------------------------------------------------------------
+ * Machine probes of the GPU (synchronous; they allocate and free their own
+ * buffers).  They replace the reference's CPU probes measure_peak_gflops /
+ * measure_bandwidth_gbs (include/tinysolverdx/tinysolverdx.hpp:31-32,
+ * src/runtime.cpp:262-310), giving the measured ceilings of the roofline.
+ *   bandwidth: TSDX_HIP_PROBE_TRIAD (c = a + 3b, 3 x 1 GiB, bytes counted 3x)
+ *              or TSDX_HIP_PROBE_COPY (c = a, bytes counted 2x); GB/s.
+ *   peak:      multiply-add loop on the vector ALUs, TSDX_HIP_PROBE_FP32 or
+ *              _FP64; GFLOP/s (fp32 without packed instructions).
+ *   mfma:      independent chains of one matrix-core instruction:
+ *              _MFMA_FP32 (16x16x4 f32), _MFMA_FP64 (16x16x4 f64),
+ *              _MFMA_BF16 / _MFMA_FP16 (16x16x32, float accumulation); TFLOP/s.
+ * ------------------------------------------------------------------------ */
+#define TSDX_HIP_PROBE_TRIAD 0
+#define TSDX_HIP_PROBE_COPY 1
+#define TSDX_HIP_PROBE_FP32 0
+#define TSDX_HIP_PROBE_FP64 1
+#define TSDX_HIP_PROBE_MFMA_FP32 0
+#define TSDX_HIP_PROBE_MFMA_FP64 1
+#define TSDX_HIP_PROBE_MFMA_BF16 2
+#define TSDX_HIP_PROBE_MFMA_FP16 3
+
+int tsdx_hip_measure_bandwidth_gbs(int kind, double *gbs);
+int tsdx_hip_measure_peak_gflops(int precision, double *gflops);
+int tsdx_hip_measure_mfma_tflops(int kind, double *tflops);
 
 /* ---------------------------------------------------------------------------
  * Host-pointer entry points (synchronous).  Shapes of the reference's CUDA

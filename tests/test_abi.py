@@ -38,7 +38,8 @@ def test_core_module_surface(tsdx):
                  "lu_solve_f32", "lu_solve_f64", "round_to_f32", "round_to_f64", "posv_f32", "posv_f64", "gesv_f32",
                  "gesv_f64", "has_cuda", "has_hip", "has_openmp", "num_threads", "set_num_threads",
                  "unit_roundoff", "chol_backward_error", "lu_backward_error", "solve_backward_error",
-                 "solve_forward_error", "reference_solve"]:
+                 "solve_forward_error", "reference_solve", "measure_peak_gflops", "measure_bandwidth_gbs",
+                 "measure_mfma_tflops"]:
         assert hasattr(core, name), name
     assert tsdx.has_cuda() is False
 
@@ -58,6 +59,11 @@ def test_abi_argument_errors_without_device():
     # batch 0 is a no-op that needs no device
     assert L.tsdx_hip_potrf_f64(None, 8, 0, None, None) == _lib.OK
     assert L.tsdx_hip_cholesky_batched_f32(p, 65, 1, None) == _lib.EINVAL
+    # machine probes: unknown probe kinds and null outputs are argument errors
+    v = C.c_double(0.0)
+    assert L.tsdx_hip_measure_bandwidth_gbs(7, C.byref(v)) == _lib.EINVAL
+    assert L.tsdx_hip_measure_peak_gflops(0, None) == _lib.EINVAL
+    assert L.tsdx_hip_measure_mfma_tflops(9, C.byref(v)) == _lib.EINVAL
     assert L.tsdx_hip_lu_batched_f64(p, 3, 1, None, None) == _lib.EINVAL
     assert L.tsdx_hip_posv_ir_workspace_size(1000) >= 4 * 1000
     assert L.tsdx_hip_posv_ir_f64(p, 65, 1, p, p, 1, 1, 10, 1e-14, None, None, None, 0, None) == _lib.EINVAL

@@ -12,7 +12,8 @@ def test_quick_lab_sweep(tmp_path):
     import tinysolverdx as tsdx
     from tinysolverdx import lab
     lab.main(["--out", str(tmp_path), "--quick", "--n", "8", "--n-tier", "32"])
-    for f in ("error_vs_cond.csv", "error_vs_n.csv", "tiers_vs_cond.csv", "REPORT.md"):
+    for f in ("error_vs_cond.csv", "error_vs_n.csv", "tiers_vs_cond.csv", "REPORT.md", "error_vs_cond_fp32.png",
+              "backward_error_vs_cond.png", "backward_error_vs_n.png", "tiers_vs_cond.png"):
         assert (tmp_path / f).exists(), f
     rows = list(csv.DictReader(open(tmp_path / "error_vs_cond.csv")))
     for r in rows:

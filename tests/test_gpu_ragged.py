@@ -1,9 +1,12 @@
 """Ragged batches: sizes 1..64 and both kinds in one graph launch.
 
-Every factor, pivot and info must be bit-identical to the per-size launches
-(tsdx_hip_potrf / tsdx_hip_getrf on a dense batch of the same matrices), which
-the plain parity tests pin to the reference.  The graph is replayed several
-times with eager work and synchronisations in between, each replay checked.
+Every factor, pivot and info must be bit-identical to the oracle (the
+reference's Auto backend restated: kernels.hpp:18-180, batched.hpp:121-244)
+run on each (kind, n) group as a dense batch, and to the per-size launches
+(tsdx_hip_potrf / tsdx_hip_getrf) of the same matrices.  The inputs include
+matrices that are not SPD, zero columns and NaNs.  The graph is replayed
+several times with eager work and synchronisations in between, each replay
+checked.
 """
 import numpy as np
 import pytest
@@ -74,6 +77,29 @@ def _per_size_reference(A, offs, sizes, kinds, poffs):
     return out, piv, info
 
 
+def _oracle_reference(oracle, A, offs, sizes, kinds, poffs):
+    """The oracle on every (kind, n) group of the ragged batch, scattered back
+    to the ragged layout."""
+    out = A.copy()
+    piv = np.zeros(int(sizes.sum()), np.int32)
+    info = np.zeros(len(sizes), np.int32)
+    for kind in (0, 1):
+        for n in np.unique(sizes[kinds == kind]):
+            n = int(n)
+            ids = np.nonzero((sizes == n) & (kinds == kind))[0]
+            dense = np.stack([A[offs[m]:offs[m] + n * n].reshape(n, n) for m in ids])
+            if kind == 0:
+                F, I = oracle.cholesky(dense)
+            else:
+                F, P, I = oracle.lu(dense)
+                for r, m in enumerate(ids):
+                    piv[poffs[m]:poffs[m] + n] = P[r]
+            for r, m in enumerate(ids):
+                out[offs[m]:offs[m] + n * n] = F[r].ravel()
+                info[m] = I[r]
+    return out, piv, info
+
+
 def _check(got, want, sizes, kinds):
     gA, gP, gI = got
     wA, wP, wI = want
@@ -85,12 +111,15 @@ def _check(got, want, sizes, kinds):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("tune", [False, True])
-def test_ragged_matches_per_size_launches(dtype, tune):
+def test_ragged_matches_oracle_and_per_size_launches(oracle, dtype, tune):
     import torch
     from tinysolverdx import device
     rng = np.random.default_rng(31 + (dtype == np.float64) + 2 * tune)
     A, offs, sizes, kinds, poffs = _batch(rng, 3000, dtype)
-    want = _per_size_reference(A, offs, sizes, kinds, poffs)
+    print(f"[ragged] {np.dtype(dtype).name} tune={tune} count={len(sizes)}", flush=True)
+    want = _oracle_reference(oracle, A, offs, sizes, kinds, poffs)
+    assert (want[2] > 0).any() and ((want[2] > 0) & (kinds == 1)).any()  # failures of both kinds are in the batch
+    _check(_per_size_reference(A, offs, sizes, kinds, poffs), want, sizes, kinds)
     dA0 = torch.from_numpy(A).cuda()
     dA = dA0.clone()
     d_off, d_poff = torch.from_numpy(offs).cuda(), torch.from_numpy(poffs).cuda()
@@ -112,6 +141,45 @@ def test_ragged_matches_per_size_launches(dtype, tune):
         d_info.fill_(-7)
         torch.cuda.synchronize()
     plan.close()
+
+
+def test_ragged_plan_keeps_its_sizes(oracle):
+    """The plan reads sizes and kinds once: changing the caller's arrays after
+    planning changes nothing in later runs."""
+    import torch
+    from tinysolverdx import device
+    rng = np.random.default_rng(77)
+    A, offs, sizes, kinds, poffs = _batch(rng, 400, np.float64)
+    want = _oracle_reference(oracle, A, offs, sizes, kinds, poffs)
+    dA0 = torch.from_numpy(A).cuda()
+    dA = dA0.clone()
+    d_size, d_kind = torch.from_numpy(sizes).cuda(), torch.from_numpy(kinds).cuda()
+    d_piv = torch.zeros(int(sizes.sum()), dtype=torch.int32, device="cuda")
+    d_info = torch.zeros(len(sizes), dtype=torch.int32, device="cuda")
+    plan = device.RaggedPlan(dA, torch.from_numpy(offs).cuda(), d_size, d_kind, d_piv,
+                             torch.from_numpy(poffs).cuda(), d_info, tune=False)
+    d_size.fill_(1)
+    d_kind.fill_(0)
+    plan.run()
+    torch.cuda.synchronize()
+    _check((dA.cpu().numpy(), d_piv.cpu().numpy(), d_info.cpu().numpy()), want, sizes, kinds)
+    plan.close()
+
+
+def test_ragged_rejects_buffers_too_small():
+    import torch
+    from tinysolverdx import device
+    A = torch.zeros(20, device="cuda")
+    sizes = torch.tensor([3, 4], dtype=torch.int32, device="cuda")
+    kinds = torch.tensor([0, 1], dtype=torch.int32, device="cuda")
+    with pytest.raises(ValueError, match="outside A"):
+        device.RaggedPlan(A, torch.tensor([0, 9], dtype=torch.int64, device="cuda"), sizes, kinds)
+    with pytest.raises(ValueError, match="outside piv"):
+        device.RaggedPlan(torch.zeros(25, device="cuda"), torch.tensor([0, 9], dtype=torch.int64, device="cuda"),
+                          sizes, kinds, torch.zeros(4, dtype=torch.int32, device="cuda"),
+                          torch.tensor([0, 3], dtype=torch.int64, device="cuda"))
+    with pytest.raises(ValueError, match="shape"):
+        device.RaggedPlan(A, torch.zeros(3, dtype=torch.int64, device="cuda"), sizes, kinds)
 
 
 def test_ragged_single_size_and_empty():

@@ -125,7 +125,7 @@ struct tsdx_hip_ragged_plan {
   int64_t count = 0;
   void* A = nullptr;
   const int64_t* off = nullptr;
-  const int* size = nullptr;
+  int* size = nullptr;  // plan-owned copy of the sizes read at creation
   int* piv = nullptr;
   const int64_t* poff = nullptr;
   int* info = nullptr;
@@ -158,6 +158,7 @@ void destroy_plan(tsdx_hip_ragged_plan* p) {
   if (p->fork) (void)hipEventDestroy(p->fork);
   if (p->cap) (void)hipStreamDestroy(p->cap);
   if (p->order) (void)hipFree(p->order);
+  if (p->size) (void)hipFree(p->size);
   if (p->counters) (void)hipFree(p->counters);
   delete p;
 }
@@ -287,7 +288,9 @@ tsdx_hip_ragged_plan* create_plan(T* A, const int64_t* off, const int* sizes, co
     p->count = count;
     p->A = A;
     p->off = off;
-    p->size = sizes;
+    // the kernels read the sizes the plan was built from, not the caller's
+    // buffer: buckets, task bounds and register classes were fixed from them
+    p->size = count > 0 ? upload(size) : nullptr;
     p->piv = piv;
     p->poff = poff;
     p->info = info;

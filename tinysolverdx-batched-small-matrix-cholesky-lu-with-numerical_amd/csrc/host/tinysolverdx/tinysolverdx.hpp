@@ -46,4 +46,15 @@ void solve_backward_error(const double* A, const double* x, const double* b, int
 void solve_forward_error(const double* A, const double* x, const double* b, int n, std::int64_t batch, double* out);
 void reference_solve(const double* A, const double* b, int n, std::int64_t batch, double* x);
 
+// Machine probes of the GPU (the reference's measure_peak_gflops /
+// measure_bandwidth_gbs measure its CPU, tinysolverdx.hpp:31-32).
+// measure_peak_gflops: multiply-add loop on the vector ALUs, float64 for FP64
+// and float otherwise (the reference's rule); measure_bandwidth_gbs: the triad
+// c = a + 3b over 1 GiB arrays in HBM; measure_mfma_tflops: one matrix-core
+// instruction of the format (FP32 16x16x4, FP64 16x16x4, BF16 / FP16 16x16x32).
+// std::runtime_error when no device is visible.
+double measure_peak_gflops(Precision p);
+double measure_bandwidth_gbs();
+double measure_mfma_tflops(Precision p);
+
 }  // namespace tsdx

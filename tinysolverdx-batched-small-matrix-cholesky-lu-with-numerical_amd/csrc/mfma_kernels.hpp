@@ -537,8 +537,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
   __shared__ __align__(16) float fs[16];
   __shared__ v4f wl[nupper<NBLK>() * kWave];
   const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
-  if (a.list && (int)blockIdx.x >= *a.lcount) return;  // uniform: the whole workgroup leaves
-  const int64_t m = a.list ? a.list[blockIdx.x] : blockIdx.x;
+  const int64_t m = a.m0 + blockIdx.x;
   double ad[nupper<NBLK>()][4];
   const double anorm = mx_load<NBLK>(a, m, g, j, ad);
   mx_solve<NBLK, LF>(a, m, g, j, ad, anorm, xs, fs, wl);
@@ -708,7 +707,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
   __shared__ v4f wl[nupper<NBLK>() * kWave];
   __shared__ __align__(16) float ul[80 * NBLK];
   const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
-  const int64_t m = blockIdx.x;
+  const int64_t m = a.m0 + blockIdx.x;
   double ad[nupper<NBLK>()][4];
   const double anorm = mx_load<NBLK>(a, m, g, j, ad);
   float f[kRF];

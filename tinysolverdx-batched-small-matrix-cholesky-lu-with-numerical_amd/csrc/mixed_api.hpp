@@ -19,11 +19,12 @@ struct IrArgs {
   int* stat;    // (batch) or null: corrections used, or -1 when sent to the float64 solve
   int* flist;   // (batch) matrices handed to the float64 solve
   int* fcount;  // (1) their number
-  // Optional subset for the refinement kernel: when list is set, workgroup w
-  // solves matrix list[w] for w < *lcount (the router's tier lists).
-  const int* list = nullptr;
-  const int* lcount = nullptr;
+  // First matrix of this launch: workgroup w of the one-wave-per-matrix
+  // kernels solves matrix m0 + w.  Large batches are split into launches of
+  // at most kIrChunk workgroups (a grid must stay below 2^32 work items).
+  int64_t m0 = 0;
 };
+constexpr int64_t kIrChunk = int64_t(1) << 24;
 
 // low: 1 float, 2 bfloat16, 3 half (refinement), 4 plain float64 for all, 5
 // the float64 fallback tier for all.

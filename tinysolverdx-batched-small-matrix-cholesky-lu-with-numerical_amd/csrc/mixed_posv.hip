@@ -21,14 +21,18 @@ __global__ void k_iota(int* list, int* count, int* stat, int64_t batch) {
 
 // Matrix-core refinement kernel: one wave (one workgroup) per matrix.
 template <int NBLK>
-static void launch_mx(const IrArgs& a, int low, hipStream_t s) {
-  const dim3 grid((unsigned)a.batch), block(kWave);
-  if (low == 2)
-    hipLaunchKernelGGL((k_posv_mx<NBLK, Low::BF16>), grid, block, 0, s, a);
-  else if (low == 3)
-    hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F16>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F32>), grid, block, 0, s, a);
+static void launch_mx(const IrArgs& a0, int low, hipStream_t s) {
+  for (int64_t m0 = 0; m0 < a0.batch; m0 += kIrChunk) {
+    IrArgs a = a0;
+    a.m0 = m0;
+    const dim3 grid((unsigned)(a0.batch - m0 < kIrChunk ? a0.batch - m0 : kIrChunk)), block(kWave);
+    if (low == 2)
+      hipLaunchKernelGGL((k_posv_mx<NBLK, Low::BF16>), grid, block, 0, s, a);
+    else if (low == 3)
+      hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F16>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_posv_mx<NBLK, Low::F32>), grid, block, 0, s, a);
+  }
 }
 
 // float64 posv of the matrices on the fallback list (grid-stride over it):

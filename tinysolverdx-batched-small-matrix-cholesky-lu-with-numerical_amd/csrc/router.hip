@@ -56,22 +56,26 @@ __global__ void k_zero_counts(int* p, int k) {
   if ((int)threadIdx.x < k) p[threadIdx.x] = 0;
 }
 
-hipError_t launch_posv_routed(const IrArgs& a, const float* P, float* Wt, int* tier, float* feat, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, a.fcount, 1);
+hipError_t launch_posv_routed(const IrArgs& a0, const float* P, float* Wt, int* tier, float* feat, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, a0.fcount, 1);
   hipLaunchKernelGGL(k_router_tiles, dim3(1), dim3(256), 0, s, P, Wt);
-  const dim3 grid((unsigned)a.batch), block(kWave);
-  const int nblk = (a.n + 15) / 16;
-  if (nblk == 1)
-    hipLaunchKernelGGL((k_posv_mx_routed<1>), grid, block, 0, s, a, P, Wt, tier, feat);
-  else if (nblk == 2)
-    hipLaunchKernelGGL((k_posv_mx_routed<2>), grid, block, 0, s, a, P, Wt, tier, feat);
-  else if (nblk == 3)
-    hipLaunchKernelGGL((k_posv_mx_routed<3>), grid, block, 0, s, a, P, Wt, tier, feat);
-  else
-    hipLaunchKernelGGL((k_posv_mx_routed<4>), grid, block, 0, s, a, P, Wt, tier, feat);
+  const int nblk = (a0.n + 15) / 16;
+  for (int64_t m0 = 0; m0 < a0.batch; m0 += kIrChunk) {
+    IrArgs a = a0;
+    a.m0 = m0;
+    const dim3 grid((unsigned)(a0.batch - m0 < kIrChunk ? a0.batch - m0 : kIrChunk)), block(kWave);
+    if (nblk == 1)
+      hipLaunchKernelGGL((k_posv_mx_routed<1>), grid, block, 0, s, a, P, Wt, tier, feat);
+    else if (nblk == 2)
+      hipLaunchKernelGGL((k_posv_mx_routed<2>), grid, block, 0, s, a, P, Wt, tier, feat);
+    else if (nblk == 3)
+      hipLaunchKernelGGL((k_posv_mx_routed<3>), grid, block, 0, s, a, P, Wt, tier, feat);
+    else
+      hipLaunchKernelGGL((k_posv_mx_routed<4>), grid, block, 0, s, a, P, Wt, tier, feat);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_posv_gather(a, s);
+  return launch_posv_gather(a0, s);
 }
 
 }  // namespace tsdx_hip

@@ -1,15 +1,18 @@
 """tinysolverdx (MI355X / HIP build): batched small-matrix Cholesky / LU on NumPy arrays.
 
-Same API as the reference package (python/tinysolverdx/__init__.py:1-117):
-inputs are (batch, n, n) arrays (a single (n, n) matrix is also accepted) with
-n in 1..64; dtype selects the precision (float32 -> fp32, float64 -> fp64).
-The work runs on the GPU (backend "auto" = "hip"); the reference's CPU
-backends ("naive", "scalar", "simd") and its emulated fp16 / bf16 precisions
-are not part of this build and raise RuntimeError.  round_to and the
+The public functions keep the reference package's interface
+(python/tinysolverdx/__init__.py:1-117): matrices come as a (batch, n, n)
+array or one (n, n) matrix, n in 1..64, and the array's dtype picks the
+precision unless `precision` is given (float32 -> "fp32", float64 -> "fp64",
+float16 -> "fp16"; "bf16" on a float32 array gives float32 results holding
+bfloat16 values).  Inputs are never modified.  The work runs on the GPU
+(backend "auto" = "hip"); the reference's CPU backends ("naive", "scalar",
+"simd") are not part of this build and raise RuntimeError.  round_to and the
 long-double error metrics run on the host, as in the reference.
 
-Differences that add to the reference's API: posv / gesv solve in one fused
-device launch; `tinysolverdx.device` takes torch device tensors without copies.
+Beyond the reference: posv / gesv run factor and solve as one device launch,
+the machine probes measure the GPU, and `tinysolverdx.device` works on torch
+device tensors without copies.
 """
 import numpy as np
 
@@ -21,114 +24,118 @@ from . import _core  # noqa: E402
 from ._core import (  # noqa: E402,F401  (re-exported)
     has_hip, has_cuda, has_openmp, num_threads, set_num_threads, unit_roundoff,
     chol_backward_error, lu_backward_error, solve_backward_error,
-    solve_forward_error, reference_solve,
+    solve_forward_error, reference_solve, measure_peak_gflops, measure_bandwidth_gbs,
+    measure_mfma_tflops,
 )
 
 __all__ = ["cholesky", "lu", "cholesky_solve", "lu_solve", "posv", "gesv", "round_to",
            "has_hip", "has_cuda", "has_openmp", "num_threads", "set_num_threads", "unit_roundoff",
            "chol_backward_error", "lu_backward_error", "solve_backward_error",
-           "solve_forward_error", "reference_solve"]
+           "solve_forward_error", "reference_solve", "measure_peak_gflops", "measure_bandwidth_gbs",
+           "measure_mfma_tflops"]
 
-_DEFAULT_PREC = {np.dtype(np.float32): "fp32", np.dtype(np.float64): "fp64", np.dtype(np.float16): "fp16"}
-_PRECISIONS = ("fp16", "bf16", "fp32", "fp64")
-
-
-def _fn(name, arr):
-    # one entry point per storage dtype, no implicit casting (reference __init__.py:25-27)
-    return getattr(_core, f"{name}_{'f64' if arr.dtype == np.float64 else 'f32'}")
+# dtype -> precision used when none is given
+_IMPLIED = {np.dtype(np.float16): "fp16", np.dtype(np.float32): "fp32", np.dtype(np.float64): "fp64"}
+_KNOWN = frozenset(("fp16", "bf16", "fp32", "fp64"))
 
 
-def _prep(a, precision, ndim_single):
-    """(contiguous work array, precision, was-single, caller dtype); reference __init__.py:30-45."""
-    a = np.asarray(a)
-    single = a.ndim == ndim_single
-    if single:
-        a = a[None]
-    out_dtype = a.dtype
-    if precision is None:
-        if a.dtype not in _DEFAULT_PREC:
-            raise TypeError(f"unsupported dtype {a.dtype}; use float16/32/64")
-        precision = _DEFAULT_PREC[a.dtype]
-    if precision not in _PRECISIONS:
-        raise ValueError("precision must be one of fp16, bf16, fp32, fp64")
-    work = np.float64 if (a.dtype == np.float64) else np.float32
-    return np.ascontiguousarray(a, dtype=work), precision, single, out_dtype
+class _Stack:
+    """One call's matrices as the C-contiguous (batch, n, n) array the core
+    takes: float64 storage stays float64 (the core rounds into a narrower
+    precision itself), everything else is stored as float32."""
 
+    def __init__(self, A, precision):
+        a = np.asarray(A)
+        self.single = a.ndim == 2
+        self.caller_dtype = a.dtype
+        if precision is None:
+            precision = _IMPLIED.get(a.dtype)
+            if precision is None:
+                raise TypeError(f"unsupported dtype {a.dtype}; use float16/32/64")
+        elif precision not in _KNOWN:
+            raise ValueError("precision must be one of fp16, bf16, fp32, fp64")
+        self.precision = precision
+        storage = np.float64 if a.dtype == np.float64 else np.float32
+        self.data = np.ascontiguousarray(a[np.newaxis] if self.single else a, dtype=storage)
+        self._suffix = "f64" if storage is np.float64 else "f32"
 
-def _finish(x, out_dtype, precision):
-    if precision == "fp16" and out_dtype == np.float16:
-        return x.astype(np.float16)
-    return x
+    def call(self, op, *args):
+        """_core.<op>_<storage>(*args)."""
+        return getattr(_core, f"{op}_{self._suffix}")(*args)
 
+    def rhs(self, B):
+        """(right-hand sides as (batch, n, k) in the matrices' storage, whether
+        the caller passed vectors).  One matrix takes b (n,) or B (n, k); a
+        stack takes (batch, n) or (batch, n, k)."""
+        b = np.asarray(B)
+        vectors = b.ndim == (1 if self.single else 2)
+        if vectors:
+            b = b[..., np.newaxis]
+        if self.single:
+            b = b[np.newaxis]
+        return np.ascontiguousarray(b, dtype=self.data.dtype), vectors
 
-def _rhs(B, ref, single):
-    B = np.asarray(B)
-    if single:
-        B = B[:, None] if B.ndim == 1 else B
-        B = B[None]
-    elif B.ndim == 2:
-        B = B[:, :, None]
-    return np.ascontiguousarray(B, dtype=ref.dtype)
+    def result(self, x, vectors=False):
+        """A solver output in the caller's terms: float16 again for fp16 work on
+        float16 input, the vector axis and the batch axis dropped where the
+        caller had none."""
+        if self.precision == "fp16" and self.caller_dtype == np.float16:
+            x = x.astype(np.float16)
+        if vectors:
+            x = x[..., 0]
+        return x[0] if self.single else x
+
+    def flag(self, x):
+        """info / piv for the caller's shape."""
+        return x[0] if self.single else x
 
 
 def cholesky(A, precision=None, backend="auto"):
     """Lower Cholesky factors and LAPACK-style info (0 = ok, k>0: pivot k not positive)."""
-    a, p, single, dt = _prep(A, precision, 2)
-    L, info = _fn("cholesky", a)(a, p, backend)
-    L = _finish(L, dt, p)
-    return (L[0], info[0]) if single else (L, info)
+    s = _Stack(A, precision)
+    L, info = s.call("cholesky", s.data, s.precision, backend)
+    return s.result(L), s.flag(info)
 
 
 def lu(A, precision=None, backend="auto"):
     """Packed LU with partial pivoting: (LU, piv, info). piv is scipy/LAPACK style (0-based)."""
-    a, p, single, dt = _prep(A, precision, 2)
-    LU, piv, info = _fn("lu", a)(a, p, backend)
-    LU = _finish(LU, dt, p)
-    return (LU[0], piv[0], info[0]) if single else (LU, piv, info)
+    s = _Stack(A, precision)
+    LU, piv, info = s.call("lu", s.data, s.precision, backend)
+    return s.result(LU), s.flag(piv), s.flag(info)
 
 
 def cholesky_solve(L, B, precision=None):
     """Solve L L^T x = B given factors from :func:`cholesky`."""
-    l, p, single, dt = _prep(L, precision, 2)
-    vec = np.asarray(B).ndim == (1 if single else 2)
-    X = _fn("cholesky_solve", l)(l, _rhs(B, l, single), p)
-    X = _finish(X, dt, p)
-    X = X[..., 0] if vec else X
-    return X[0] if single else X
+    s = _Stack(L, precision)
+    b, vectors = s.rhs(B)
+    return s.result(s.call("cholesky_solve", s.data, b, s.precision), vectors)
 
 
 def lu_solve(LU, piv, B, precision=None):
     """Solve A x = B given (LU, piv) from :func:`lu`."""
-    l, p, single, dt = _prep(LU, precision, 2)
-    pv = np.ascontiguousarray(np.asarray(piv)[None] if single else piv, dtype=np.int32)
-    vec = np.asarray(B).ndim == (1 if single else 2)
-    X = _fn("lu_solve", l)(l, pv, _rhs(B, l, single), p)
-    X = _finish(X, dt, p)
-    X = X[..., 0] if vec else X
-    return X[0] if single else X
+    s = _Stack(LU, precision)
+    p = np.asarray(piv)
+    p = np.ascontiguousarray(p[np.newaxis] if s.single else p, dtype=np.int32)
+    b, vectors = s.rhs(B)
+    return s.result(s.call("lu_solve", s.data, p, b, s.precision), vectors)
 
 
 def posv(A, B, precision=None, backend="auto"):
     """Solve SPD systems A x = B (factor and solve fused on the device). Returns (x, info)."""
-    a, p, single, dt = _prep(A, precision, 2)
-    vec = np.asarray(B).ndim == (1 if single else 2)
-    _, X, info = _fn("posv", a)(a, _rhs(B, a, single), p, backend)
-    X = _finish(X, dt, p)
-    X = X[..., 0] if vec else X
-    return (X[0], info[0]) if single else (X, info)
+    s = _Stack(A, precision)
+    b, vectors = s.rhs(B)
+    _, X, info = s.call("posv", s.data, b, s.precision, backend)
+    return s.result(X, vectors), s.flag(info)
 
 
 def gesv(A, B, precision=None, backend="auto"):
     """Solve general systems A x = B with partial pivoting (fused on the device). Returns (x, info)."""
-    a, p, single, dt = _prep(A, precision, 2)
-    vec = np.asarray(B).ndim == (1 if single else 2)
-    _, _, X, info = _fn("gesv", a)(a, _rhs(B, a, single), p, backend)
-    X = _finish(X, dt, p)
-    X = X[..., 0] if vec else X
-    return (X[0], info[0]) if single else (X, info)
+    s = _Stack(A, precision)
+    b, vectors = s.rhs(B)
+    _, _, X, info = s.call("gesv", s.data, b, s.precision, backend)
+    return s.result(X, vectors), s.flag(info)
 
 
 def round_to(x, precision):
     """Round values to the nearest number representable in ``precision`` (float64 in/out)."""
-    x = np.ascontiguousarray(x, dtype=np.float64)
-    return _core.round_to_f64(x, precision)
+    return _core.round_to_f64(np.ascontiguousarray(x, dtype=np.float64), precision)

@@ -16,10 +16,10 @@ the GPU computes:
 
     python -m tinysolverdx.lab --out DIR [--quick]
 
-writes error_vs_cond.csv, error_vs_n.csv, tiers_vs_cond.csv and REPORT.md.
+writes error_vs_cond.csv, error_vs_n.csv, tiers_vs_cond.csv, REPORT.md and the PNG charts
+of write_plots.
 """
 import argparse
-import csv
 import os
 import pathlib
 
@@ -32,40 +32,55 @@ PLAIN = ["fp32", "fp64"]
 TIERS = ["fp32+IR", "bf16+IR", "fp16+IR", "routed"]
 
 
-def _med(v, ok):
-    v = v[ok]
-    return float(np.median(v)) if v.size else np.nan
+def _stat(values, keep, how):
+    """median or max of values[keep] (nan when nothing is kept)."""
+    v = np.asarray(values)[keep]
+    if v.size == 0:
+        return np.nan
+    return float(np.median(v) if how == "median" else np.max(v))
 
 
-def _max(v, ok):
-    v = v[ok]
-    return float(np.max(v)) if v.size else np.nan
+# kind -> (matrix generator, factor + backward error, fused solve), all on the GPU
+def _chol(A, prec):
+    L, info = tsdx.cholesky(A, precision=prec, backend="hip")
+    return info, tsdx.chol_backward_error(A, L.astype(np.float64))
+
+
+def _lu(A, prec):
+    LU, piv, info = tsdx.lu(A, precision=prec, backend="hip")
+    return info, tsdx.lu_backward_error(A, LU.astype(np.float64), piv)
+
+
+KINDS = {
+    "chol": (matrices.spd, _chol, tsdx.posv),
+    "lu": (matrices.general, _lu, tsdx.gesv),
+}
 
 
 def run_plain_point(rng, prec, kappa, n, trials):
-    """One row per kind for (precision, kappa, n): factor and solve on the GPU."""
-    rows = []
+    """Rows (one per kind) for one (precision, kappa, n): the inputs are
+    rounded to `prec` first, so every precision factors the matrix it can hold;
+    factor and solve run on the GPU, the metrics on the host in long double."""
     b = rng.standard_normal((trials, n))
-    for kind, gen in (("chol", matrices.spd), ("lu", matrices.general)):
-        A = tsdx.round_to(gen(rng, trials, n, kappa), prec)  # the matrix actually factored
+    rows = []
+    for kind, (make, factor, solve) in KINDS.items():
+        A = tsdx.round_to(make(rng, trials, n, kappa), prec)
+        info, bw = factor(A, prec)
+        x = solve(A, b, precision=prec, backend="hip")[0].astype(np.float64)
+        good = (info == 0) & np.isfinite(bw)               # factorization succeeded
+        solved = good & np.all(np.isfinite(x), axis=1)     # and gave a finite solution
+        xs = np.where(solved[:, None], x, 0.0)
+        fwd = tsdx.solve_forward_error(A, xs, b)
+        sbw = tsdx.solve_backward_error(A, xs, b)
         cond = np.linalg.cond(A)
-        if kind == "chol":
-            L, info = tsdx.cholesky(A, precision=prec, backend="hip")
-            bw = tsdx.chol_backward_error(A, L.astype(np.float64))
-            x, _ = tsdx.posv(A, b, precision=prec, backend="hip")
-        else:
-            LU, piv, info = tsdx.lu(A, precision=prec, backend="hip")
-            bw = tsdx.lu_backward_error(A, LU.astype(np.float64), piv)
-            x, _ = tsdx.gesv(A, b, precision=prec, backend="hip")
-        ok = (info == 0) & np.isfinite(bw)
-        x = x.astype(np.float64)
-        okx = ok & np.isfinite(x).all(axis=1)
-        x0 = np.where(okx[:, None], x, 0.0)
-        fwd = tsdx.solve_forward_error(A, x0, b)
-        sbw = tsdx.solve_backward_error(A, x0, b)
-        rows.append(dict(precision=prec, kind=kind, n=n, kappa=kappa, cond_actual=_med(cond, np.isfinite(cond)),
-                         fail_frac=float(1 - ok.mean()), backward_med=_med(bw, ok), backward_max=_max(bw, ok),
-                         solve_backward_med=_med(sbw, okx), forward_med=_med(fwd, okx), forward_max=_max(fwd, okx)))
+        rows.append({
+            "precision": prec, "kind": kind, "n": n, "kappa": kappa,
+            "cond_actual": _stat(cond, np.isfinite(cond), "median"),
+            "fail_frac": float(np.count_nonzero(~good)) / trials,
+            "backward_med": _stat(bw, good, "median"), "backward_max": _stat(bw, good, "max"),
+            "solve_backward_med": _stat(sbw, solved, "median"),
+            "forward_med": _stat(fwd, solved, "median"), "forward_max": _stat(fwd, solved, "max"),
+        })
     return rows
 
 
@@ -92,15 +107,89 @@ def run_tier_point(rng, tier, kappa, n, trials):
     fwd = tsdx.solve_forward_error(A, x, b)
     return dict(tier=tier, n=n, kappa=kappa, fallback_frac=float((st < 0).mean()),
                 corrections_mean=float(st[st >= 0].mean()) if (st >= 0).any() else np.nan,
-                solve_backward_med=_med(sbw, ok), solve_backward_max=_max(sbw, ok),
-                forward_med=_med(fwd, ok), forward_max=_max(fwd, ok), info_gt0=int((~ok).sum()))
+                solve_backward_med=_stat(sbw, ok, "median"), solve_backward_max=_stat(sbw, ok, "max"),
+                forward_med=_stat(fwd, ok, "median"), forward_max=_stat(fwd, ok, "max"), info_gt0=int((~ok).sum()))
 
 
 def write_csv(path, rows):
-    with open(path, "w", newline="") as f:
-        w = csv.DictWriter(f, fieldnames=list(rows[0].keys()))
-        w.writeheader()
-        w.writerows(rows)
+    """rows (dicts with one key set) as CSV, columns in the order of the first row."""
+    cols = list(rows[0])
+    lines = [",".join(cols)]
+    lines += [",".join(repr(r[c]) if isinstance(r[c], float) else str(r[c]) for c in cols) for r in rows]
+    pathlib.Path(path).write_text("\n".join(lines) + "\n")
+
+
+def _curve(rows, x, y, **match):
+    """(x values, y values) of the rows matching every key=value, sorted by x."""
+    sel = sorted((r for r in rows if all(r[k] == v for k, v in match.items())), key=lambda r: r[x])
+    return np.array([r[x] for r in sel], float), np.array([r[y] for r in sel], float)
+
+
+def write_plots(out, cond_rows, size_rows, tier_rows, n_fixed, n_tier, size_kappa):
+    """PNG charts of the three tables (the reference's lab draws the same
+    quantities, python/tinysolverdx/lab.py:88-143):
+      error_vs_cond_<prec>.png   backward / forward error and failure rate vs kappa
+      backward_error_vs_cond.png backward error vs kappa, every precision, per kind
+      backward_error_vs_n.png    backward error vs n
+      tiers_vs_cond.png          posv tiers: backward error and share sent to float64"""
+    import matplotlib
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+
+    precs = sorted({r["precision"] for r in cond_rows}, key=lambda p: tsdx.unit_roundoff(p), reverse=True)
+    marks = {"chol": ("o", "-", "Cholesky"), "lu": ("s", "--", "LU")}
+    palette = dict(zip(precs, plt.cm.viridis(np.linspace(0.1, 0.85, max(len(precs), 1)))))
+
+    for prec in precs:
+        u = tsdx.unit_roundoff(prec)
+        fig, ax = plt.subplots(1, 3, figsize=(14, 4))
+        for kind, (mk, ls, label) in marks.items():
+            for a, key in zip(ax, ("backward_med", "forward_med", "fail_frac")):
+                k, v = _curve(cond_rows, "kappa", key, precision=prec, kind=kind)
+                (a.semilogx if key == "fail_frac" else a.loglog)(k, v, ls, marker=mk, label=label)
+        kk = np.array(sorted({r["kappa"] for r in cond_rows}))
+        ax[0].axhline(u, color="gray", ls=":", label=f"u = {u:.1e}")
+        ax[1].loglog(kk, np.minimum(kk * u, 1.0), color="gray", ls=":", label="kappa u")
+        for a, t in zip(ax, ("backward error (median)", "solve forward error (median)", "share with info > 0")):
+            a.set_title(f"{prec} on the GPU: {t}")
+            a.set_xlabel("kappa_2")
+            a.grid(alpha=0.3, which="both")
+            a.legend(fontsize=8)
+        fig.tight_layout()
+        fig.savefig(out / f"error_vs_cond_{prec}.png", dpi=120)
+        plt.close(fig)
+
+    for name, rows, x, xlabel in (("backward_error_vs_cond.png", cond_rows, "kappa", f"kappa_2 (n={n_fixed})"),
+                                  ("backward_error_vs_n.png", size_rows, "n", f"n (kappa={size_kappa:g})")):
+        fig, ax = plt.subplots(1, 2, figsize=(11, 4), sharey=True)
+        for a, kind in zip(ax, marks):
+            for prec in precs:
+                xs, ys = _curve(rows, x, "backward_med", precision=prec, kind=kind)
+                a.loglog(xs, ys, "-", marker=marks[kind][0], color=palette[prec], label=prec)
+            a.set_title(f"{marks[kind][2]}: median backward error")
+            a.set_xlabel(xlabel)
+            a.grid(alpha=0.3, which="both")
+            a.legend(fontsize=8)
+        fig.tight_layout()
+        fig.savefig(out / name, dpi=120)
+        plt.close(fig)
+
+    fig, ax = plt.subplots(1, 2, figsize=(11, 4))
+    for t in TIERS:
+        k, bw = _curve(tier_rows, "kappa", "solve_backward_max", tier=t)
+        ax[0].loglog(k, bw, "-o", label=t)
+        k, fb = _curve(tier_rows, "kappa", "fallback_frac", tier=t)
+        ax[1].semilogx(k, fb, "-o", label=t)
+    ax[0].axhline(1e-14, color="gray", ls=":", label="tolerance 1e-14")
+    ax[0].set_title(f"posv tiers, n={n_tier}: worst backward error")
+    ax[1].set_title(f"posv tiers, n={n_tier}: share solved in float64")
+    for a in ax:
+        a.set_xlabel("kappa_2")
+        a.grid(alpha=0.3, which="both")
+        a.legend(fontsize=8)
+    fig.tight_layout()
+    fig.savefig(out / "tiers_vs_cond.png", dpi=120)
+    plt.close(fig)
 
 
 def fmt(v):
@@ -163,6 +252,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=16, help="matrix size for the condition sweep of the plain kernels")
     ap.add_argument("--n-tier", type=int, default=64, help="matrix size for the posv tiers")
     ap.add_argument("--quick", action="store_true", help="fewer trials / points")
+    ap.add_argument("--no-plots", action="store_true", help="tables only")
     a = ap.parse_args(argv)
     out = pathlib.Path(a.out)
     out.mkdir(parents=True, exist_ok=True)
@@ -184,6 +274,8 @@ def main(argv=None):
     write_csv(out / "error_vs_n.csv", size_rows)
     write_csv(out / "tiers_vs_cond.csv", tier_rows)
     write_report(out / "REPORT.md", cond_rows, size_rows, tier_rows, a.n, a.n_tier, size_kappa, trials, a.seed)
+    if not a.no_plots:
+        write_plots(out, cond_rows, size_rows, tier_rows, a.n, a.n_tier, size_kappa)
     print(f"wrote report to {out}/")
 
 

@@ -29,7 +29,7 @@ stored next to the parameters (router_default.json).
 import numpy as np
 
 from . import _lib
-from .device import _chk, _shape, _stream
+from .device import Call, _stream
 
 FEATURES, HIDDEN, TIERS = 12, 64, 3
 PARAMS = 5211
@@ -45,9 +45,9 @@ TIER_NAMES = ("bf16+IR", "fp32+IR", "fp64")
 def features(A):
     """(batch, 12) float32 features of float64 A (batch, n, n), on A's device."""
     import torch
-    batch, n = _shape(A)
-    feat = torch.empty((batch, FEATURES), dtype=torch.float32, device=A.device)
-    _lib.call("tsdx_hip_router_features_f64", _chk(A, "A", torch.float64), n, batch, _chk(feat, "feat"), _stream(A))
+    c = Call(A, dtypes=(torch.float64,))
+    feat = torch.empty((c.batch, FEATURES), dtype=torch.float32, device=A.device)
+    _lib.call("tsdx_hip_router_features_f64", c.mat(), c.n, c.batch, feat.data_ptr(), c.stream())
     return feat
 
 
@@ -223,12 +223,23 @@ class Router:
     # ---------------------------------------------------------------- inference
     def forward(self, feat):
         import torch
-        count = feat.shape[0]
-        logits = torch.empty((count, TIERS), dtype=torch.float32, device=feat.device)
-        if count:
-            _lib.call("tsdx_hip_router_forward", _chk(self.params, "params", torch.float32),
-                      _chk(feat, "feat", torch.float32), count, _chk(logits, "logits"), _stream(feat))
+        c = self._features(feat)
+        logits = torch.empty((c.batch, TIERS), dtype=torch.float32, device=feat.device)
+        if c.batch:
+            _lib.call("tsdx_hip_router_forward", self._params(c), c.mat(), c.batch, logits.data_ptr(), c.stream())
         return logits
+
+    def _features(self, feat):
+        """Checks a (count, 12) float32 feature tensor on the parameters' device."""
+        import torch
+        c = Call(feat, "feat", dtypes=(torch.float32,), square=False)
+        if feat.dim() != 2 or feat.shape[1] != FEATURES:
+            raise ValueError(f"feat must have shape (count, {FEATURES}), got {tuple(feat.shape)}")
+        return c
+
+    def _params(self, c):
+        import torch
+        return c.tensor(self.params, "params", torch.float32, (PARAMS,))
 
     def predict(self, feat):
         return self.forward(feat).argmax(dim=1).to(dtype=__import__("torch").int32)
@@ -237,15 +248,15 @@ class Router:
     def grad(self, feat, label, work=None):
         """(gradient, mean cross-entropy) over the batch."""
         import torch
-        count = feat.shape[0]
+        c = self._features(feat)
+        need = _lib.lib().tsdx_hip_router_grad_workspace_size(c.batch)
         if work is None:
-            work = torch.empty(_lib.lib().tsdx_hip_router_grad_workspace_size(count), dtype=torch.uint8,
-                               device=feat.device)
+            work = torch.empty(need, dtype=torch.uint8, device=feat.device)
+        w, wn = c.work(work, need)
         g = torch.empty(PARAMS, dtype=torch.float32, device=feat.device)
         loss = torch.empty(1, dtype=torch.float32, device=feat.device)
-        _lib.call("tsdx_hip_router_grad", _chk(self.params, "params", torch.float32), _chk(feat, "feat", torch.float32),
-                  _chk(label, "label", torch.int32), count, _chk(g, "grad"), _chk(loss, "loss"), _chk(work, "work"),
-                  work.numel(), _stream(feat))
+        _lib.call("tsdx_hip_router_grad", self._params(c), c.mat(), c.ints(label, "label", optional=False), c.batch,
+                  g.data_ptr(), loss.data_ptr(), w, wn, c.stream())
         return g, loss
 
     def calibrate(self, cost):
@@ -290,8 +301,8 @@ class Router:
                 f, l = feat[idx].contiguous(), label[idx].contiguous()
                 g, loss = self.grad(f, l, work)
                 step += 1
-                _lib.call("tsdx_hip_router_adam", _chk(self.params, "params"), _chk(g, "grad"), _chk(m, "m"),
-                          _chk(v, "v"), step, float(lr), _stream(f))
+                _lib.call("tsdx_hip_router_adam", self.params.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(),
+                          step, float(lr), _stream(f))
                 losses.append(loss)
             history.append(float(torch.cat(losses).mean().item()))
         return history
@@ -322,7 +333,12 @@ class Router:
         self.direct every matrix takes the float64 tier without the network
         (tier 2, stat -1, feat left untouched)."""
         import torch
-        batch, n = _shape(A)
+        c = Call(A, dtypes=(torch.float64,))
+        batch, n = c.batch, c.n
+        b, nrhs = c.rhs(B)
+        x = c.like(X, B)
+        pinfo, pstat, ptier = c.ints(info, "info"), c.ints(stat, "stat"), c.ints(tier, "tier")
+        pfeat = c.tensor(feat, "feat", torch.float32, (batch, FEATURES), optional=True)
         if bucket(n) in self.direct:
             from . import device
             if n <= 32:  # the float64 tier there is the plain float64 posv: one launch, no list
@@ -337,14 +353,12 @@ class Router:
             if tier is not None:
                 tier.fill_(2)
             return X, info, stat, tier
+        need = _lib.lib().tsdx_hip_posv_routed_workspace_size(batch)
         if work is None:
-            work = torch.empty(_lib.lib().tsdx_hip_posv_routed_workspace_size(batch), dtype=torch.uint8,
-                               device=A.device)
-        _lib.call("tsdx_hip_posv_routed_f64", _chk(A, "A", torch.float64), n, batch, _chk(B, "B", torch.float64),
-                  _chk(X, "X", torch.float64), B.shape[-1], _chk(self.params, "params", torch.float32), int(max_iter),
-                  float(tol), _chk(info, "info", torch.int32), _chk(stat, "stat", torch.int32),
-                  _chk(tier, "tier", torch.int32), _chk(feat, "feat", torch.float32), _chk(work, "work"),
-                  work.numel(), _stream(A))
+            work = torch.empty(need, dtype=torch.uint8, device=A.device)
+        w, wn = c.work(work, need)
+        _lib.call("tsdx_hip_posv_routed_f64", c.mat(), n, batch, b, x, nrhs, self._params(c), int(max_iter),
+                  float(tol), pinfo, pstat, ptier, pfeat, w, wn, c.stream())
         return X, info, stat, tier
 
 
