@@ -22,27 +22,28 @@ bool has_cuda() { return false; }
 
 // ------------------------------------------------------- machine probes
 namespace {
-double probe(int rc, double v) {
-  if (rc != TSDX_HIP_OK) throw std::runtime_error(tsdx_hip_last_error());
+// Runs one probe of the C ABI (rc = f(&value)) and returns its value.
+template <class F>
+double probe(F&& f) {
+  double v = 0.0;
+  if (f(&v) != TSDX_HIP_OK) throw std::runtime_error(tsdx_hip_last_error());
   return v;
 }
 }  // namespace
 
 double measure_peak_gflops(Precision p) {
-  double v = 0.0;
-  return probe(tsdx_hip_measure_peak_gflops(p == Precision::FP64 ? TSDX_HIP_PROBE_FP64 : TSDX_HIP_PROBE_FP32, &v), v);
+  const int kind = p == Precision::FP64 ? TSDX_HIP_PROBE_FP64 : TSDX_HIP_PROBE_FP32;
+  return probe([&](double* v) { return tsdx_hip_measure_peak_gflops(kind, v); });
 }
 double measure_bandwidth_gbs() {
-  double v = 0.0;
-  return probe(tsdx_hip_measure_bandwidth_gbs(TSDX_HIP_PROBE_TRIAD, &v), v);
+  return probe([](double* v) { return tsdx_hip_measure_bandwidth_gbs(TSDX_HIP_PROBE_TRIAD, v); });
 }
 double measure_mfma_tflops(Precision p) {
   const int kind = p == Precision::FP64   ? TSDX_HIP_PROBE_MFMA_FP64
                    : p == Precision::BF16 ? TSDX_HIP_PROBE_MFMA_BF16
                    : p == Precision::FP16 ? TSDX_HIP_PROBE_MFMA_FP16
                                           : TSDX_HIP_PROBE_MFMA_FP32;
-  double v = 0.0;
-  return probe(tsdx_hip_measure_mfma_tflops(kind, &v), v);
+  return probe([&](double* v) { return tsdx_hip_measure_mfma_tflops(kind, v); });
 }
 
 // ------------------------------------------------------------- threads

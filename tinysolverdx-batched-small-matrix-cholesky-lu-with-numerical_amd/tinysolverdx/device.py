@@ -79,13 +79,13 @@ class Call:
     def mat(self):
         return self.A.data_ptr()
 
-    def ints(self, t, name, per_matrix=1, optional=True):
-        """int32 (batch,) or (batch, per_matrix)."""
-        shape = (self.batch,) if per_matrix == 1 else (self.batch, per_matrix)
-        return self.tensor(t, name, _torch().int32, shape, optional)
+    def ints(self, t, name, optional=True):
+        """int32 (batch,)."""
+        return self.tensor(t, name, _torch().int32, (self.batch,), optional)
 
     def piv(self, t, optional=False):
-        return self.ints(t, "piv", self.n, optional)
+        """int32 (batch, n)."""
+        return self.tensor(t, "piv", _torch().int32, (self.batch, self.n), optional)
 
     def rhs(self, B, name="B"):
         """(batch, n, nrhs) of the matrices' dtype; returns (address, nrhs)."""
