@@ -126,6 +126,20 @@ size_t tsdx_hip_posv_ir_workspace_size(int64_t batch);
 int tsdx_hip_posv_ir_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs, int factor,
                          int max_iter, double tol, int *info, int *stat, void *work, size_t work_bytes, void *stream);
 
+/* Allocation-free tsdx_hip_posv_ir_f64.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as tsdx_hip_posv_ir_f64 without the workspace: the
+ * fallback list lives in a fixed device array of the library (16 MiB + 256
+ * bytes on every GPU it runs on, held until the process ends), so batch may
+ * not exceed TSDX_HIP_NOALLOC_MAX_BATCH (TSDX_HIP_EINVAL otherwise).  The call
+ * allocates nothing.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (other streams, other threads) race and must not be
+ * issued. */
+#define TSDX_HIP_NOALLOC_MAX_BATCH (1 << 22)
+int tsdx_hip_posv_ir_noalloc_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs,
+                                 int factor, int max_iter, double tol, int *info, int *stat, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Learned precision router for posv (no reference counterpart).  A small
  * network picks, per matrix, the cheapest tier expected to reach the

@@ -33,6 +33,11 @@ void hip_check(hipError_t e, const char* what);
 void check_dims(int n, int64_t batch);
 void require_device();
 
+// Refinement-tier posv over a caller's workspace (capi.cpp).
+size_t ir_workspace_size(int64_t batch);
+void posv_ir(const double* A, int n, int64_t batch, const double* B, double* X, int nrhs, int factor, int max_iter,
+             double tol, int* info, int* stat, void* work, size_t work_bytes, hipStream_t s);
+
 // Runs f and maps exceptions to the ABI's return codes.
 template <class F>
 int guarded(F&& f) {

@@ -39,6 +39,7 @@ for _t in ("f32", "f64"):
     })
 SIGNATURES["tsdx_hip_posv_ir_workspace_size"] = [_i64]
 SIGNATURES["tsdx_hip_posv_ir_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, C.c_size_t, _vp]
+SIGNATURES["tsdx_hip_posv_ir_noalloc_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp]
 for _t in ("f32", "f64"):
     SIGNATURES[f"tsdx_hip_ragged_plan_create_{_t}"] = [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i, C.POINTER(_vp), _vp]
 SIGNATURES["tsdx_hip_ragged_run"] = [_vp, _vp]
