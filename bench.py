@@ -79,6 +79,30 @@ def time_steps(fn, steps, warmup):
     return wall, e0.elapsed_time(e1) * 1e-3
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def bit_mismatches(got, want):
+    """Number of matrices whose factor differs from `want` in any bit (every
+    NaN counts as equal to every NaN: NaN payloads differ between hosts)."""
+    import numpy as np
+    g = np.ascontiguousarray(got).reshape(got.shape[0], -1)
+    w = np.ascontiguousarray(want).reshape(want.shape[0], -1)
+    iv = {4: np.uint32, 8: np.uint64}[g.dtype.itemsize]
+    gb, wb = g.view(iv).copy(), w.view(iv).copy()
+    gb[np.isnan(g)] = 0
+    wb[np.isnan(w)] = 0
+    return int(((gb != wb) | (np.isnan(g) != np.isnan(w))).any(axis=1).sum())
+
+
 def cpu_baseline(A, B, nrhs):
     """The reference's posv (its Auto backend: SIMD Cholesky + scalar solve),
     compiled from its own sources, on the host cores; best of reps like the
@@ -99,7 +123,7 @@ def cpu_baseline(A, B, nrhs):
         dt = time.perf_counter() - t0
         best, total, reps = min(best, dt), total + dt, reps + 1
     batch = Ah.shape[0]
-    return {"value": batch / best, "unit": "solves/s", "cores": cores, "kind": kind,
+    return {"value": batch / best, "unit": "solves/s", "cores": cores, "kind": kind, "cpu_model": cpu_model(),
             "sample": f"full workload ({batch} matrices, nrhs={nrhs}), best of {reps} reps"}
 
 
@@ -336,7 +360,8 @@ def _cpu_timed(fn, units, unit, sample):
         dt = time.perf_counter() - t0
         best, total, reps = min(best, dt), total + dt, reps + 1
     return {"value": units / best, "unit": unit, "cores": len(os.sched_getaffinity(0)),
-            "kind": "reference" if available("reference") else "port", "sample": f"{sample}, best of {reps} reps"}
+            "kind": "reference" if available("reference") else "port", "cpu_model": cpu_model(),
+            "sample": f"{sample}, best of {reps} reps"}
 
 
 def _cpu_oracle():
@@ -380,17 +405,24 @@ def run_potrf(a):
     ms = ev / a.steps * 1e3
     per = 2 * 4 * n * n + 4
     log(f"[bench] potrf fp32 n={n}: {ms:.3f} ms/launch, info>0: {bad}, max ||A-LL^T||/||A|| (256 sampled) {res:.2e}")
+    # parity over the whole batch: the timed run's factors and info against
+    # the reference's CPU backend (compiled from its sources, or the oracle)
+    orc = _cpu_oracle()
+    Ah = A0.cpu().numpy()
+    L0, i0 = orc.cholesky(Ah)
+    parity = {"checked": batch, "factor_bit_mismatches": bit_mismatches(A.cpu().numpy(), L0),
+              "info_mismatches": int((info.cpu().numpy() != i0).sum()), "against": orc.kind}
+    log(f"[bench] potrf parity vs {orc.kind}: {parity}")
     cpu = None
     if not a.no_cpu:
-        orc = _cpu_oracle()
-        Ah = A0.cpu().numpy()
         cpu = _cpu_timed(lambda: orc.cholesky(Ah), batch, "matrices/s", f"full workload ({batch} matrices)")
     return _line("potrf matrices/s fp32 n=32 (65536 SPD); HBM GB/s", batch / (ms * 1e-3), "matrices/s", a, ms,
                  "f32", {"workload": f"potrf fp32 n={n}, batch={batch}, Q diag(lambda) Q^T, lambda log-uniform "
                                      "[1e-3,1]", "info_gt0": bad, "max_rel_residual_sampled": res,
+                         "parity": parity,
                          "gflops": round(batch / (ms * 1e-3) * n ** 3 / 3 / 1e9, 1),
                          "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)},
-                 _roof(per * batch, ms * 1e-3, "k_potrf_rows<float,8,4,1,1,false,true>", per,
+                 _roof(per * batch, ms * 1e-3, "k_potrf_rows<float,8,4,1,1,false,true,false>", per,
                        f"potrf_f32_n{n}_b{batch}"), cpu)
 
 
@@ -423,14 +455,23 @@ def run_getrf(a):
     per_s = es * n * n + 2 * es * n * 4 + 4 * n + 4
     log(f"[bench] getrf {a.dtype} n={n}: {ms:.3f} ms, gesv(4) {ms_s:.3f} ms, info>0: {bad} "
         f"(zeroed columns {zc.numel()})")
+    # parity over the whole batch: pivots, info and factors of the timed run
+    # against the reference's CPU backend, bit for bit
+    orc = _cpu_oracle()
+    Afull = A0.cpu().numpy()
+    LU0, p0, i0 = orc.lu(Afull)
+    parity = {"checked": batch, "pivot_mismatches": int((piv.cpu().numpy() != p0).any(axis=1).sum()),
+              "info_mismatches": int((info.cpu().numpy() != i0).sum()),
+              "factor_bit_mismatches": bit_mismatches(A.cpu().numpy(), LU0), "against": orc.kind}
+    del Afull, LU0
+    log(f"[bench] getrf parity vs {orc.kind}: {parity}")
     cpu = None
     if not a.no_cpu:
-        orc = _cpu_oracle()
         take = min(batch, max(1024, int(4e8 // (n ** 3))))  # a bounded sample
         Ah = A0[:take].cpu().numpy()
         cpu = _cpu_timed(lambda: orc.lu(Ah), take, "matrices/s", f"first {take} of the {batch} matrices")
     cfg = {"workload": f"getrf {a.dtype} n={n}, batch={batch} (~1 GiB of A), N(0,1), 1% zeroed columns",
-           "info_gt0": bad, "zeroed_columns": int(zc.numel()),
+           "info_gt0": bad, "zeroed_columns": int(zc.numel()), "parity": parity,
            "gflops": round(batch / (ms * 1e-3) * 2 * n ** 3 / 3 / 1e9, 1),
            "gesv4_matrices_per_s": round(batch / (ms_s * 1e-3), 1),
            "gesv4_gflops": round(batch / (ms_s * 1e-3) * (2 * n ** 3 / 3 + 2 * n * n * 4) / 1e9, 1),
@@ -444,9 +485,10 @@ def run_getrf(a):
 def _getrf_kernel(es, n, batch):
     """Name of the kernel getrf launches for this element size, n and batch."""
     nb = max(4, 1 << (n - 1).bit_length())
+    ztile = "true" if n <= 24 else "false"  # the reference's zero-pivot rule by n (tile kernel up to 24)
     if es == 4 and nb == 64 and batch >= 16384:  # two rows per lane (csrc/rows_kernels.hpp, GetrfShape)
-        return "k_getrf_rows<float,32,2,1>"
-    return f"k_getrf<{'float' if es == 4 else 'double'},{nb}>"
+        return f"k_getrf_rows<float,32,2,1,{ztile}>"
+    return f"k_getrf<{'float' if es == 4 else 'double'},{nb},{ztile}>"
 
 
 def ragged_inputs(count, dtype, seed):
@@ -507,9 +549,15 @@ def run_ragged(a):
     n64 = sizes.astype(np.int64)
     total = int((2 * es * n64 * n64 + 4 * n64 * kinds + 4).sum())
     bad = int((info != 0).sum().item())
-    # the same matrices as per-size launches (one launch per kind and n), for comparison
+    # the same matrices as per-size launches (one launch per kind and n), for
+    # comparison of time and, bit for bit, of every factor, pivot and info;
+    # and every (kind, n) group against the reference's CPU backend
     per_ms = 0.0
     s = torch.cuda.current_stream()
+    got_A, got_piv, got_info = A.cpu().numpy(), piv.cpu().numpy(), info.cpu().numpy()
+    orc = _cpu_oracle()
+    Ah0 = A0.cpu().numpy()
+    mism = {"per_size": 0, "oracle": 0}
     for kind in (0, 1):
         for n in range(2, 65):
             ids = np.nonzero((sizes == n) & (kinds == kind))[0]
@@ -528,12 +576,33 @@ def run_ragged(a):
             e1.record(s)
             torch.cuda.synchronize()
             per_ms += e0.elapsed_time(e1)
-    log(f"[bench] ragged {a.dtype}: {ms:.3f} ms per graph launch, per-size launches {per_ms:.3f} ms, info>0: {bad}")
+            gi = (offs[ids][:, None] + np.arange(n * n)[None, :])
+            pi = (poffs[ids][:, None] + np.arange(n)[None, :])
+            mine = (got_A[gi], got_info[ids], got_piv[pi] if kind == 1 else None)
+            ref_dev = (D.cpu().numpy().reshape(ids.size, -1), I.cpu().numpy(), P.cpu().numpy() if kind == 1 else None)
+            if kind == 0:
+                Lo, Io = orc.cholesky(Ah0[gi].reshape(ids.size, n, n))
+                ref_cpu = (Lo.reshape(ids.size, -1), Io, None)
+            else:
+                Lo, Po, Io = orc.lu(Ah0[gi].reshape(ids.size, n, n))
+                ref_cpu = (Lo.reshape(ids.size, -1), Io, Po)
+            for key, ref in (("per_size", ref_dev), ("oracle", ref_cpu)):
+                bad_m = np.zeros(ids.size, bool)
+                g_u = mine[0].view({4: np.uint32, 8: np.uint64}[mine[0].dtype.itemsize]).copy()
+                r_u = ref[0].view(g_u.dtype).copy()
+                g_u[np.isnan(mine[0])] = 0
+                r_u[np.isnan(ref[0])] = 0
+                bad_m |= ((g_u != r_u) | (np.isnan(mine[0]) != np.isnan(ref[0]))).any(axis=1)
+                bad_m |= mine[1] != ref[1]
+                if kind == 1:
+                    bad_m |= (mine[2] != ref[2]).any(axis=1)
+                mism[key] += int(bad_m.sum())
+    log(f"[bench] ragged {a.dtype}: {ms:.3f} ms per graph launch, per-size launches {per_ms:.3f} ms, info>0: {bad}, "
+        f"matrices differing from the per-size launches / the {orc.kind} CPU backend: {mism}")
     cpu = None
     if not a.no_cpu:
-        orc = _cpu_oracle()
         take = min(count, 100_000)
-        Ah = A0.cpu().numpy()
+        Ah = Ah0
         groups = []
         for kind in (0, 1):
             for n in range(2, 65):
@@ -549,6 +618,8 @@ def run_ragged(a):
     cfg = {"workload": f"ragged {a.dtype}: {count} matrices, n Zipf on 2..64 (P(n) ~ 1/n), half SPD Cholesky "
                        "(kappa log-uniform [1,1e4]) half N(0,1) LU, one graph launch",
            "plan": {k: v for k, v in plan.report.items()}, "info_gt0": bad,
+           "parity": {"checked": count, "mismatches_vs_per_size_launches": mism["per_size"],
+                      "mismatches_vs_cpu_reference": mism["oracle"], "cpu_reference": orc.kind},
            "per_size_launches_ms": round(per_ms, 4), "graph_over_per_size": round(per_ms / ms, 3),
            "gflops": round(float(((n64.astype(np.float64) ** 3) * np.where(kinds == 1, 2.0 / 3.0, 1.0 / 3.0)).sum())
                            / (ms * 1e-3) / 1e9, 1),

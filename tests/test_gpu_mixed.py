@@ -39,7 +39,7 @@ def _bwd(tsdx, A, X, B):
 
 
 @pytest.mark.parametrize("factor", ["fp32", "bf16", "fp16", "fp64", "fp64_block"])
-@pytest.mark.parametrize("n,nrhs", [(64, 1), (64, 4), (32, 2), (16, 8), (5, 1)])
+@pytest.mark.parametrize("n,nrhs", [(64, 1), (64, 4), (64, 8), (32, 2), (16, 8), (5, 1)])
 def test_every_answer_meets_tolerance(tsdx, factor, n, nrhs):
     rng = np.random.default_rng(n * 10 + nrhs)
     A, B, kappa = _system(rng, 96, n, 1, 6, nrhs)

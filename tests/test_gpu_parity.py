@@ -220,6 +220,57 @@ def test_getrf_full_bucket_batches_bit_exact(hip, oracle, dt, n, batch):
     assert bool((buf[0] == 7.0).all()) and bool((buf[batch + 1] == 7.0).all())
 
 
+@pytest.mark.parametrize("dt,n", [(np.float32, 8), (np.float32, 16), (np.float32, 32), (np.float32, 64),
+                                  (np.float64, 8), (np.float64, 16), (np.float64, 32), (np.float64, 64)])
+def test_getrf_config1_full_scale_bit_exact(hip, oracle, dt, n):
+    """The LU workload at its full size: ~1 GiB of N(0,1) matrices per (dtype,
+    n) (fp32: 4194304 / 1048576 / 262144 / 65536 matrices for n = 8 / 16 / 32 /
+    64, fp64 half as many), 1% of them with one zeroed column.  Pivots, info
+    and factors bit-exact against the oracle for every matrix (pivot choices
+    are sensitive to the fused multiply-add chain at about 1 in 4000 fp32
+    n=64 matrices, so only the full batch shows them all); the fused gesv with
+    4 right-hand sides reproduces the same pivots and info and solves to the
+    reference's accuracy."""
+    import torch
+    from tinysolverdx import device
+    es = np.dtype(dt).itemsize
+    batch = (1 << 30) // (es * n * n)
+    rng = np.random.default_rng(4000 + n + es)
+    print(f"[getrf config-1 scale] {np.dtype(dt).name} n={n} batch={batch}", flush=True)
+    G = rng.standard_normal((batch, n, n), dtype=np.float32 if dt == np.float32 else np.float64)
+    zc = rng.choice(batch, batch // 100, replace=False)
+    G[zc, :, rng.integers(0, n, zc.size)] = 0.0
+    tdt = torch.float32 if dt == np.float32 else torch.float64
+    dA0 = torch.from_numpy(G).cuda()
+    dG = dA0.clone()
+    piv = torch.zeros((batch, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    device.getrf(dG, piv, info)
+    torch.cuda.synchronize()
+    LUo, pivo, io = oracle.lu(G)
+    got_i, got_p = info.cpu().numpy(), piv.cpu().numpy()
+    assert np.array_equal(got_i, io), int((got_i != io).sum())
+    assert np.array_equal(got_p, pivo), int((got_p != pivo).any(axis=1).sum())
+    assert bits_equal(dG.cpu().numpy(), LUo), first_bit_mismatch(dG.cpu().numpy(), LUo)
+    assert (io > 0).sum() >= zc.size // 2
+    del LUo, dG
+    B = torch.randn((batch, n, 4), generator=torch.Generator(device="cuda").manual_seed(n), device="cuda", dtype=tdt)
+    X = torch.empty_like(B)
+    piv2 = torch.zeros_like(piv)
+    info2 = torch.zeros_like(info)
+    device.gesv_solve(dA0, B, X, piv2, info2)
+    torch.cuda.synchronize()
+    assert torch.equal(piv2, piv) and torch.equal(info2, info)
+    ok = info == 0
+    A64, X64, B64 = dA0[ok].double(), X[ok].double(), B[ok].double()
+    r = torch.linalg.vector_norm(B64 - A64 @ X64, dim=1)
+    den = torch.linalg.matrix_norm(A64)[:, None] * torch.linalg.vector_norm(X64, dim=1) + \
+        torch.linalg.vector_norm(B64, dim=1)
+    bw = (r / den).max().item()
+    u = 2.0 ** -24 if dt == np.float32 else 2.0 ** -53
+    assert bw < 8 * n * u, bw
+
+
 @pytest.mark.parametrize("dt,n,batch", [(np.float32, 32, 65536), (np.float32, 13, 131072), (np.float32, 50, 16384),
                                          (np.float64, 29, 32768), (np.float32, 32, 65531), (np.float32, 16, 65539),
                                          (np.float64, 32, 32771), (np.float32, 64, 16387), (np.float32, 8, 8195),
