@@ -558,6 +558,7 @@ def run_ragged(a):
     orc = _cpu_oracle()
     Ah0 = A0.cpu().numpy()
     mism = {"per_size": 0, "oracle": 0}
+    outs = []
     for kind in (0, 1):
         for n in range(2, 65):
             ids = np.nonzero((sizes == n) & (kinds == kind))[0]
@@ -576,27 +577,32 @@ def run_ragged(a):
             e1.record(s)
             torch.cuda.synchronize()
             per_ms += e0.elapsed_time(e1)
-            gi = (offs[ids][:, None] + np.arange(n * n)[None, :])
-            pi = (poffs[ids][:, None] + np.arange(n)[None, :])
-            mine = (got_A[gi], got_info[ids], got_piv[pi] if kind == 1 else None)
-            ref_dev = (D.cpu().numpy().reshape(ids.size, -1), I.cpu().numpy(), P.cpu().numpy() if kind == 1 else None)
-            if kind == 0:
-                Lo, Io = orc.cholesky(Ah0[gi].reshape(ids.size, n, n))
-                ref_cpu = (Lo.reshape(ids.size, -1), Io, None)
-            else:
-                Lo, Po, Io = orc.lu(Ah0[gi].reshape(ids.size, n, n))
-                ref_cpu = (Lo.reshape(ids.size, -1), Io, Po)
-            for key, ref in (("per_size", ref_dev), ("oracle", ref_cpu)):
-                bad_m = np.zeros(ids.size, bool)
-                g_u = mine[0].view({4: np.uint32, 8: np.uint64}[mine[0].dtype.itemsize]).copy()
-                r_u = ref[0].view(g_u.dtype).copy()
-                g_u[np.isnan(mine[0])] = 0
-                r_u[np.isnan(ref[0])] = 0
-                bad_m |= ((g_u != r_u) | (np.isnan(mine[0]) != np.isnan(ref[0]))).any(axis=1)
-                bad_m |= mine[1] != ref[1]
-                if kind == 1:
-                    bad_m |= (mine[2] != ref[2]).any(axis=1)
-                mism[key] += int(bad_m.sum())
+            outs.append((kind, n, ids, D, I, P))
+    # parity after all launches are timed (long host work between timed
+    # launches would let the clocks drop)
+    for kind, n, ids, D, I, P in outs:
+        gi = (offs[ids][:, None] + np.arange(n * n)[None, :])
+        pi = (poffs[ids][:, None] + np.arange(n)[None, :])
+        mine = (got_A[gi], got_info[ids], got_piv[pi] if kind == 1 else None)
+        ref_dev = (D.cpu().numpy().reshape(ids.size, -1), I.cpu().numpy(), P.cpu().numpy() if kind == 1 else None)
+        if kind == 0:
+            Lo, Io = orc.cholesky(Ah0[gi].reshape(ids.size, n, n))
+            ref_cpu = (Lo.reshape(ids.size, -1), Io, None)
+        else:
+            Lo, Po, Io = orc.lu(Ah0[gi].reshape(ids.size, n, n))
+            ref_cpu = (Lo.reshape(ids.size, -1), Io, Po)
+        for key, ref in (("per_size", ref_dev), ("oracle", ref_cpu)):
+            bad_m = np.zeros(ids.size, bool)
+            g_u = mine[0].view({4: np.uint32, 8: np.uint64}[mine[0].dtype.itemsize]).copy()
+            r_u = ref[0].view(g_u.dtype).copy()
+            g_u[np.isnan(mine[0])] = 0
+            r_u[np.isnan(ref[0])] = 0
+            bad_m |= ((g_u != r_u) | (np.isnan(mine[0]) != np.isnan(ref[0]))).any(axis=1)
+            bad_m |= mine[1] != ref[1]
+            if kind == 1:
+                bad_m |= (mine[2] != ref[2]).any(axis=1)
+            mism[key] += int(bad_m.sum())
+    del outs
     log(f"[bench] ragged {a.dtype}: {ms:.3f} ms per graph launch, per-size launches {per_ms:.3f} ms, info>0: {bad}, "
         f"matrices differing from the per-size launches / the {orc.kind} CPU backend: {mism}")
     cpu = None

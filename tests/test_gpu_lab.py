@@ -29,3 +29,18 @@ def test_quick_lab_sweep(tmp_path):
     # fp32 refinement converges without the float64 solve while kappa * u_fp32 << 1
     easy = [r for r in tiers if r["tier"] == "fp32+IR" and float(r["kappa"]) <= 1e4]
     assert easy and all(float(r["fallback_frac"]) == 0.0 for r in easy)
+
+
+def test_quick_bench_sweep(tmp_path):
+    """tinysolverdx.bench: the per-size sweep writes the reference's CSV
+    columns and the roofline chart, and its numbers sit under the measured
+    ceilings."""
+    from tinysolverdx import bench
+    rows = bench.main(["--out", str(tmp_path), "--quick"])
+    assert (tmp_path / "bench_fp32.csv").exists() and (tmp_path / "roofline_fp32.png").exists()
+    got = list(csv.DictReader(open(tmp_path / "bench_fp32.csv")))
+    assert {"kind", "n", "precision", "series", "gflops", "batch", "threads"} <= set(got[0])
+    assert {r["series"] for r in got} == set(bench.SERIES)
+    for r in rows:
+        assert np.isfinite(r["gflops"]) and r["gflops"] > 0
+        assert r["frac_of_measured_hbm"] < 1.5, r  # algorithmic bytes cannot beat the stream rate by much
