@@ -647,7 +647,7 @@ def run_router(a):
     from tinysolverdx import device, router as rt
     total = a.batch if a.batch != 131072 else 1 << 20
     t0 = time.perf_counter()
-    feat, label = rt.training_set(count_per_n=20000, seed=20261208)
+    feat, label = rt.training_set(seed=20261208)
     R = rt.Router(seed=3)
     hist = R.fit(feat, label, epochs=60, batch=4096, lr=3e-3, seed=3)
     train_acc = float((R.predict(feat) == label).float().mean().item())
@@ -699,7 +699,7 @@ def run_router(a):
     log(f"[bench] router: routed {rate['routed'] / 1e6:.2f} M/s, un-routed fp32 {rate['unrouted'] / 1e6:.2f} M/s, "
         f"fp64 {rate['fp64'] / 1e6:.2f} M/s, tiers {tiers.tolist()}, wasted attempts {wasted}")
     cfg = {"workload": f"routed posv, {count} SPD fp64 matrices, n uniform on {{16,32,64}}, kappa log-uniform "
-                       "[1,1e12], nrhs=1; router trained in this run on 3 x 20000 separately seeded matrices",
+                       "[1,1e12], nrhs=1; router trained in this run on the shipped training mix (tinysolverdx.router.TRAINING_MIX), separately seeded",
            "train_loss_first_last": [round(hist[0], 4), round(hist[-1], 4)], "train_accuracy": round(train_acc, 4),
            "direct_float64_buckets": direct, "fp64_logit_shift": round(shift, 4),
            "train_labels_bf16_fp32_fp64": np.bincount(label.cpu().numpy(), minlength=3).tolist(),

@@ -161,6 +161,16 @@ int tsdx_hip_posv_ir_noalloc_f64(const double *A, int n, int64_t batch, const do
 
 int tsdx_hip_router_features_f64(const double *A, int n, int64_t batch, float *feat, void *stream);
 int tsdx_hip_router_forward(const float *params, const float *feat, int64_t count, float *logits, void *stream);
+/* Inference as the routed posv runs it: the network on the matrix cores with
+ * bfloat16 weights and activations and float accumulation (the feature
+ * normalisation folded into the first layer), so its tiers are the routed
+ * solve's choices for the same features; logits within bfloat16 rounding of
+ * tsdx_hip_router_forward's.  tier (count) and / or logits (count, 3); work:
+ * tsdx_hip_router_infer_workspace_size() bytes of device memory (16-byte
+ * aligned) for the prepared weights. */
+size_t tsdx_hip_router_infer_workspace_size(void);
+int tsdx_hip_router_infer(const float *params, const float *feat, int64_t count, int *tier, float *logits, void *work,
+                          size_t work_bytes, void *stream);
 size_t tsdx_hip_router_grad_workspace_size(int64_t count);
 int tsdx_hip_router_grad(const float *params, const float *feat, const int *label, int64_t count, float *grad,
                          float *loss, void *work, size_t work_bytes, void *stream);
@@ -173,9 +183,10 @@ int tsdx_hip_router_adam(float *params, const float *grad, float *m, float *v, i
  * may be NULL, else they receive the network's choice and the features (the
  * same definitions as tsdx_hip_router_features_f64).  Only the lower triangle
  * of A is read.  work: device workspace of
- * tsdx_hip_posv_routed_workspace_size(batch) bytes (the fallback list and a
- * per-call copy of the first two weight layers); calls sharing one workspace
- * must not overlap. */
+ * tsdx_hip_posv_routed_workspace_size(batch) bytes, 16-byte aligned (the
+ * float64 and bfloat16 tier lists and the network prepared for the matrix
+ * cores, rewritten on every call); calls sharing one workspace must not
+ * overlap.  The network runs as in tsdx_hip_router_infer. */
 size_t tsdx_hip_posv_routed_workspace_size(int64_t batch);
 int tsdx_hip_posv_routed_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs,
                              const float *params, int max_iter, double tol, int *info, int *stat, int *tier,

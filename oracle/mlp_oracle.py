@@ -120,3 +120,43 @@ def features(A):
         f[11] = float((d > r).mean())
         out.append(f)
     return np.array(out)
+
+
+def round_bf16(x):
+    """float32 values rounded to the nearest bfloat16 (ties to even), as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32)
+
+
+def infer_bf16(p, feat):
+    """Logits of the matrix-core inference path (tsdx_hip_router_infer): the
+    feature normalisation folded into layer 1 in float32 (W1' = W1 / scale,
+    b1' = b1 - sum_q W1'[:, q] mean_q in the order of q), weights, inputs and
+    hidden activations rounded to bfloat16, sums in float64 here (the matrix
+    cores accumulate bfloat16 products exactly in float32 in an order of their
+    own, so GPU logits agree to float32 rounding of the sums)."""
+    p32 = np.asarray(p, np.float32)
+    mu, sd = p32[0:12], p32[12:24]
+    W1 = p32[OFF_W1:OFF_B1].reshape(H, F)
+    W1f = (W1 / sd[None, :]).astype(np.float32)
+    b1f = p32[OFF_B1:OFF_W2].copy()
+    for q in range(F):
+        b1f = (b1f - (W1f[:, q] * mu[q]).astype(np.float32)).astype(np.float32)
+    W2 = p32[OFF_W2:OFF_B2].reshape(H, H)
+    W3 = p32[OFF_W3:OFF_B3].reshape(T, H)
+    b2, b3 = p32[OFF_B2:OFF_W3], p32[OFF_B3:PARAMS]
+    x = round_bf16(np.asarray(feat, np.float32)).astype(np.float64)
+    a1 = x @ round_bf16(W1f).astype(np.float64).T + b1f
+    h1 = round_bf16(np.maximum(a1, 0.0).astype(np.float32)).astype(np.float64)
+    a2 = h1 @ round_bf16(W2).astype(np.float64).T + b2
+    h2 = round_bf16(np.maximum(a2, 0.0).astype(np.float32)).astype(np.float64)
+    return h2 @ round_bf16(W3).astype(np.float64).T + b3
+
+
+def tiers(z):
+    """First largest logit; a NaN logit gives tier 2."""
+    z = np.asarray(z)
+    t = np.argmax(z, axis=1)
+    t[np.isnan(z).any(axis=1)] = 2
+    return t.astype(np.int32)

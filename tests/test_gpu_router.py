@@ -61,6 +61,32 @@ def test_forward_and_gradient_match_oracle():
         assert abs(loss.item() - M.loss(p, feat, label)) < 1e-4
 
 
+def test_matrix_core_inference_matches_oracle():
+    """tsdx_hip_router_infer (bfloat16 network on the matrix cores, the
+    routed solve's decision path) against the oracle's restatement, and its
+    logits within bfloat16 rounding of the float32 forward pass."""
+    import torch
+    import mlp_oracle as M
+    from tinysolverdx import router
+    rng = np.random.default_rng(12)
+    p = router.init_params(5)
+    p[0:12] = rng.standard_normal(12)
+    p[12:24] = rng.uniform(0.5, 2, 12)
+    p[router.OFF_B1:router.OFF_W2] = rng.standard_normal(64) * 0.1
+    r = router.Router(p)
+    for count in (1, 15, 16, 17, 1000):  # partial and several 16-sample columns
+        feat = (rng.standard_normal((count, 12)) * 2).astype(np.float32)
+        f = torch.from_numpy(feat).cuda()
+        tier, z = r.infer(f)
+        z, tier = z.cpu().numpy(), tier.cpu().numpy()
+        zo = M.infer_bf16(p, feat)
+        assert np.allclose(z, zo, rtol=1e-5, atol=1e-5 * np.abs(zo).max()), np.abs(z - zo).max()
+        top2 = np.sort(zo, axis=1)[:, -2:]
+        clear = (top2[:, 1] - top2[:, 0]) > 1e-4 * np.abs(zo).max()
+        assert np.array_equal(tier[clear], M.tiers(zo)[clear])
+        assert np.allclose(z, r.forward(f).cpu().numpy(), rtol=5e-2, atol=5e-2 * np.abs(zo).max())
+
+
 def test_adam_step_matches_oracle():
     import torch
     import mlp_oracle as M
@@ -194,8 +220,13 @@ def test_fused_features_match_standalone_kernel():
         dA = torch.from_numpy(A).cuda()
         B = torch.from_numpy(rng.standard_normal((24, n, 1))).cuda()
         feat = torch.zeros((24, 12), dtype=torch.float32, device="cuda")
-        router.Router().posv(dA, B, torch.zeros_like(B), feat=feat)
+        tier = torch.full((24,), -1, dtype=torch.int32, device="cuda")
+        r = router.default()
+        r.direct = frozenset()
+        r.posv(dA, B, torch.zeros_like(B), feat=feat, tier=tier)
         got = feat.cpu().numpy()
+        # the routed kernel's choice is the inference path's on its features
+        assert np.array_equal(tier.cpu().numpy(), r.predict(feat).cpu().numpy()), n
         assert np.allclose(got, router.features(dA).cpu().numpy(), rtol=1e-4, atol=2e-4), n
         assert np.allclose(got, M.features(A), rtol=1e-4, atol=2e-4), n
 

@@ -209,3 +209,38 @@ def test_router_oracle_features_closed_forms():
     f = M.features(a[None])[0]
     assert np.isclose(f[4], np.log10(4.0)) and np.isclose(f[5], (2.0 - 1.0) / np.sqrt(4 + 9 + 2)) and f[11] == 1.0
     assert M.features(np.array([[[-1.0, 0.0], [0.0, 1.0]]]))[0][10] == 1.0
+
+
+def test_router_oracle_bf16_inference_pinned():
+    """infer_bf16 (the matrix-core inference path) agrees with the float64
+    forward pass to the bfloat16 rounding of its hidden activations, with the
+    identity normalisation and bfloat16 weights and features as well as with a
+    folded normalisation (a wrong fold shows as errors of order one); with
+    zero second and third layers it returns b3 exactly; bfloat16 rounding is
+    pinned by known values (ties to even)."""
+    import mlp_oracle as M
+    rb = M.round_bf16(np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -9, -2.5, 1.0 + 2.0 ** -7 + 2.0 ** -8,
+                                3.0e38], np.float32))
+    assert rb.tolist() == [1.0, 1.0, 1.0 + 2.0 ** -7, -2.5, 1.0 + 2.0 ** -6, float(M.round_bf16(np.float32(3.0e38)))]
+    assert np.isinf(M.round_bf16(np.array([3.4e38], np.float32))[0])  # past the largest bfloat16: infinity
+    rng = np.random.default_rng(9)
+    p = np.zeros(M.PARAMS, np.float32)
+    p[12:24] = 1.0
+    p[M.OFF_W1:] = M.round_bf16((rng.standard_normal(M.PARAMS - M.OFF_W1) * 0.3).astype(np.float32))
+    feat = M.round_bf16(rng.standard_normal((50, 12)).astype(np.float32))
+    # hidden activations rounded to bfloat16 make the two differ by at most
+    # 2^-9 relative per activation; with exact bf16 activations they agree:
+    z_ref = M.forward(p, feat)[3]
+    z_bf = M.infer_bf16(p, feat)
+    assert np.allclose(z_bf, z_ref, rtol=3e-2, atol=3e-2)
+    # one hidden layer of zero width (W2 = W3 = 0): logits are exactly b3
+    q = p.copy()
+    q[M.OFF_W2:M.OFF_B2] = 0.0
+    q[M.OFF_W3:M.OFF_B3] = 0.0
+    assert np.array_equal(M.infer_bf16(q, feat), np.tile(q[M.OFF_B3:].astype(np.float64), (50, 1)))
+    assert M.tiers(np.array([[1.0, 2.0, 2.0], [0.0, 0.0, 0.0], [1.0, np.nan, 0.0]])).tolist() == [1, 0, 2]
+    r = p.copy()
+    r[0:12] = rng.standard_normal(12)
+    r[12:24] = rng.uniform(0.5, 2.0, 12)
+    raw = (feat * r[12:24] + r[0:12]).astype(np.float32)  # features whose normalised values are `feat`
+    assert np.allclose(M.infer_bf16(r, raw), M.forward(r, raw)[3], rtol=3e-2, atol=3e-2)

@@ -21,9 +21,10 @@ void check_count(int64_t count) {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-// routed workspace: [fallback count (256 B)] [fallback list] [regrouped weights]
+// routed workspace: [float64 count, bfloat16 count (256 B)] [float64 list]
+// [bfloat16 list] [prepared network]
 size_t routed_list_bytes(int64_t batch) { return align256((batch < 0 ? 0 : (size_t)batch) * sizeof(int)); }
-size_t routed_workspace(int64_t batch) { return 256 + routed_list_bytes(batch) + align256(kRouterTileBytes); }
+size_t routed_workspace(int64_t batch) { return 256 + 2 * routed_list_bytes(batch) + align256(kRouterPrepBytes); }
 
 }  // namespace
 
@@ -47,6 +48,21 @@ int tsdx_hip_router_forward(const float* params, const float* feat, int64_t coun
     if (!params || !feat || !logits) throw InvalidArg("params, feat and logits must not be null");
     require_device();
     hip_check(launch_router_forward(params, feat, count, logits, S(stream)), "router forward launch");
+  });
+}
+
+size_t tsdx_hip_router_infer_workspace_size(void) { return align256(kRouterPrepBytes); }
+
+int tsdx_hip_router_infer(const float* params, const float* feat, int64_t count, int* tier, float* logits, void* work,
+                          size_t work_bytes, void* stream) {
+  return guarded([&] {
+    check_count(count);
+    if (count == 0) return;
+    if (!params || !feat || (!tier && !logits)) throw InvalidArg("params, feat and tier or logits must not be null");
+    if (!work || work_bytes < kRouterPrepBytes) throw InvalidArg("workspace missing or too small");
+    if (reinterpret_cast<uintptr_t>(work) % 16 != 0) throw InvalidArg("workspace must be 16-byte aligned");
+    require_device();
+    hip_check(launch_router_infer(params, feat, count, tier, logits, work, S(stream)), "router inference launch");
   });
 }
 
@@ -104,8 +120,11 @@ int tsdx_hip_posv_routed_f64(const double* A, int n, int64_t batch, const double
     a.stat = stat;
     a.flist = flist;
     a.fcount = fcount;
-    float* wt = reinterpret_cast<float*>(w + 256 + routed_list_bytes(batch));
-    hip_check(launch_posv_routed(a, params, wt, tier, feat, S(stream)), "routed posv launch");
+    a.lcount = fcount + 1;
+    a.llist = reinterpret_cast<int*>(w + 256 + routed_list_bytes(batch));
+    if (reinterpret_cast<uintptr_t>(work) % 16 != 0) throw InvalidArg("workspace must be 16-byte aligned");
+    void* prep = w + 256 + 2 * routed_list_bytes(batch);
+    hip_check(launch_posv_routed(a, params, prep, tier, feat, S(stream)), "routed posv launch");
   });
 }
 

@@ -41,6 +41,7 @@
 
 #include "mixed_kernels.hpp"
 #include "router_layout.hpp"
+#include "router_net.hpp"
 
 namespace tsdx_hip {
 
@@ -221,14 +222,18 @@ TSDX_BCAST_FMAC4(15)
 //
 // With U = D T (D = diag(U), T unit upper, T[k][j] = W_k[k][j] / W_k[k][k]
 // for the partly reduced W_k), step k does
-//     W[i][j] -= W[i][k] T[k][j]              (i, j > k: the Schur update)
+//     W[i][j] -= W[i][k] T[k][j]              (the Schur update)
 //     F[:,j]  -= T[k][j] F[:,k]   (j > k)     (F = identity at the start)
 // which leaves F = T^-1; V = T^-1 D^-1 scales column j by 1/U[j][j] at the end.
-// The update of W is applied to every entry: rows and columns <= k then hold
-// garbage, but W is only read afterwards at row k', column k' and the
-// diagonal (k' > k) of the live part, and the garbage only ever reaches rows
-// or columns <= k'; W itself is dropped at the end.  Only the update of F
-// needs T[k][j] masked to j > k.
+//
+// float (F32): both updates use T[k][j] masked to j > k, so column j of W is
+// last changed at step j - 1 and keeps its pivot W_j[j][j] on the diagonal to
+// the end; 1/U[j][j] = 1/sqrt(W[j][j]) is formed once after the last step
+// instead of step by step, and each step needs one reciprocal of its pivot.
+// Rows <= k of the columns > k still take the update and hold garbage, which
+// is never read again (row k' is read at step k' only, right of its pivot).
+// The low formats (BF16, F16) round every entry of U, which needs the square
+// root at each step; they keep the unmasked Schur update of W.
 template <Low LF>
 __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
   // fresh lane indices per call: the lane masks of the steps are formed where
@@ -244,24 +249,29 @@ __device__ __forceinline__ v4f diag_inverse(v4f W, int g, int j, bool& bad) {
     // (the element is copied to a scalar first: __builtin_bit_cast applied to
     // an element of a vector type reads element 0)
     const float wkv = W[kv];
+    float nt;  // -T[k][j] (masked to j > k)
     const float akk = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wkv), 16 * kg + k));
-    const float rd = __builtin_amdgcn_rsqf(akk);
     const float wk = bperm(wkv, 16 * kg + j);  // W[k][j], from row k (lane 16 kg + j)
-    float nt;                                  // -T[k][j]
     if constexpr (LF == Low::F32) {
-      nt = wk * (-rd * rd);
+      nt = (j > k) ? wk * -__builtin_amdgcn_rcpf(akk) : 0.0f;
       bcast_fmac4<k>(W, nt);
     } else {  // U entries rounded to the low format
+      const float rd = __builtin_amdgcn_rsqf(akk);
       const float uj = round_low(wk * rd, LF);
-      nt = uj * -rd;
+      nt = (j > k) ? uj * -rd : 0.0f;
       static_for<0, 4>([&](auto v) {
         const float ui = round_low(row_bcast<k>(W[v()]) * rd, LF);
         W[v()] = __builtin_fmaf(-ui, uj, W[v()]);
       });
+      rdl = (j == k) ? rd : rdl;
     }
-    rdl = (j == k) ? rd : rdl;
-    bcast_fmac4<k>(F, (j > k) ? nt : 0.0f);
+    bcast_fmac4<k>(F, nt);
   });
+  if constexpr (LF == Low::F32) {
+    // pivot j sits in register j % 4 of lane (j / 4, j)
+    const float dv = (j & 3) == 0 ? W[0] : (j & 3) == 1 ? W[1] : (j & 3) == 2 ? W[2] : W[3];
+    rdl = __builtin_amdgcn_rsqf(bperm(dv, 16 * (j >> 2) + j));
+  }
   // every pivot positive and finite <=> every 1/sqrt positive and finite
   bad = bad || __builtin_amdgcn_ballot_w64(!(rdl > 0.0f && rdl < __builtin_huge_valf())) != 0;
   static_for<0, 4>([&](auto v) { F[v()] *= rdl; });
@@ -531,6 +541,24 @@ __device__ __forceinline__ void mx_solve(const IrArgs& a, int64_t m, int g, int 
   }
 }
 
+// The same over the matrices of a list (the routed solve's bfloat16 tier):
+// workgroups stride over llist[0 .. *lcount).
+template <int NBLK, Low LF>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void k_posv_mx_list(IrArgs a) {
+  __shared__ __align__(16) double xs[16 * NBLK];
+  __shared__ __align__(16) float fs[16];
+  __shared__ v4f wl[nupper<NBLK>() * kWave];
+  const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
+  const int count = *a.lcount;
+  for (int w = blockIdx.x; w < count; w += gridDim.x) {  // uniform
+    const int64_t m = a.llist[w];
+    double ad[nupper<NBLK>()][4];
+    const double anorm = mx_load<NBLK>(a, m, g, j, ad);
+    mx_solve<NBLK, LF>(a, m, g, j, ad, anorm, xs, fs, wl);
+    wave_sync();  // the next matrix reuses the LDS buffers
+  }
+}
+
 template <int NBLK, Low LF>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void k_posv_mx(IrArgs a) {
   __shared__ __align__(16) double xs[16 * NBLK];
@@ -546,20 +574,21 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
 // ------------------------------------------------------------ in-wave router
 // The 12 router features (definitions in router_kernels.hpp, k_router_features)
 // of the symmetric matrix held in ad; every lane gets every feature.
-// Off-diagonal |a| sums are accumulated per lane in float64 as the blocks are
-// held (the diagonal left out, so r_i needs no cancellation), rounded to float
-// and transposed through LDS so that lane i ends up with row i's r_i and
-// a_ii; the per-row work and the reductions then run once over 64 lanes.
-//   tl: 256 NBLK floats of LDS, ul: 80 NBLK floats of LDS.
+// |a| sums per row r_i (diagonal left out) without moving the blocks: a stored
+// block (I, J), I <= J, adds its column sums (over the lane's four rows, then
+// the four row groups) to rows 16 J + j, and, for I < J, its row sums (over
+// the 16 lanes of a row group: transposed butterfly) to rows 16 I + 4 g + v;
+// by symmetry the two cover every entry of every row once.  The per-row work
+// then runs with one row per lane (lane (g, j): row 16 g + j).
+//   dl: 64 floats of LDS (the diagonal).
 template <int NBLK>
 __device__ __forceinline__ void mx_features(const double (&ad)[nupper<NBLK>()][4], int n, int g, int j, double anorm,
-                                            float (&f)[kRF], float* tl, float* ul) {
-  constexpr int NR = 16 * NBLK;  // rows of the padded matrix
-  const int lane = 16 * g + j;
-  // p[v]: row 16I + 4g + v over this lane's column of the blocks (I, J >= I);
-  // pc[J]: column 16J + j over rows 4g + v of the blocks (I < J), i.e. by
-  // symmetry row 16J + j over the blocks left of the diagonal.
+                                            float (&f)[kRF], float* dl) {
+  // block row by block row: the row sums of block row I are complete after
+  // its blocks and are reduced at once (few registers live next to ad)
+  const int src_mixed = 16 * (j >> 2) + 4 * (j & 3);
   double pc[NBLK];
+  float rr[NBLK];  // row 16 I + j (column form): |a| right of the diagonal block
   static_for<0, NBLK>([&](auto J) { pc[J] = 0.0; });
   static_for<0, NBLK>([&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
@@ -568,31 +597,34 @@ __device__ __forceinline__ void mx_features(const double (&ad)[nupper<NBLK>()][4
       constexpr int J = decltype(Jc)::value, b = bslot<NBLK>(I, J);
       static_for<0, 4>([&](auto v) {
         const double x = __builtin_fabs(ad[b][v()]);
-        if constexpr (I == J) {
-          p[v] += (4 * g + v == j) ? 0.0 : x;
-        } else {
-          p[v] += x;
-          pc[J] += x;
-        }
+        pc[J] += x;
+        if constexpr (I < J) p[v] += x;
       });
     });
-    // tl[j][row]: the partial sums of lane column j
-    const float4 q = {(float)p[0], (float)p[1], (float)p[2], (float)p[3]};
-    *reinterpret_cast<float4*>(tl + j * NR + 16 * I + 4 * g) = q;
+    if constexpr (I + 1 < NBLK) {
+      const float q[4] = {(float)p[0], (float)p[1], (float)p[2], (float)p[3]};
+      rr[I] = bperm(tsum4(q, j), src_mixed);
+    } else {
+      rr[I] = 0.0f;
+    }
+  });
+  // the diagonal: the lane (j / 4, j) of block (J, J) holds a_{16J+j} in register j % 4
+  static_for<0, NBLK>([&](auto J) {
     static_for<0, 4>([&](auto v) {
-      if (4 * g + v == j) ul[4 * NR + 16 * I + j] = (float)ad[bslot<NBLK>(I, I)][v()];
+      if (4 * g + v == j) dl[16 * J + j] = (float)ad[bslot<NBLK>(J, J)][v];
     });
   });
-  static_for<0, NBLK>([&](auto J) { ul[g * NR + 16 * J + j] = (float)pc[J]; });
+  float rowsum = 0.0f;  // sum_c |a_rc| of this lane's row r = 16 g + j, diagonal included
+  static_for<0, NBLK>([&](auto J) {
+    const float s = sum_g((float)pc[J]) + rr[J];
+    rowsum = (g == J) ? s : rowsum;
+  });
   wave_sync();
-  // lane = row index
-  const bool act = lane < n;  // n <= NR
-  const int e = act ? lane : 0;
-  float r = 0.0f;
-  static_for<0, 16>([&](auto jj) { r += tl[jj * NR + e]; });
-  static_for<0, 4>([&](auto gg) { r += ul[gg * NR + e]; });
-  const float d = ul[4 * NR + e];
+  const int row = 16 * g + j;
+  const bool act = row < n;
+  const float d = dl[act ? row : 0];
   wave_sync();
+  const float r = rowsum - __builtin_fabsf(d);
   const float big = 3.0e38f;
   const float ra = r + __builtin_fabsf(d);
   float dmin = act ? d : big;
@@ -602,32 +634,32 @@ __device__ __forceinline__ void mx_features(const double (&ad)[nupper<NBLK>()][4
   float dom = act ? (d > 0.0f ? r * __builtin_amdgcn_rcpf(d) : big) : 0.0f;
   float dsum = act ? d : 0.0f;
   float lsum = act ? flog10(d) : 0.0f;
-  float nonpos = (act && !(d > 0.0f)) ? 1.0f : 0.0f;
-  float ddom = (act && d > r) ? 1.0f : 0.0f;
-  // reductions over the 64 lanes, the nine in lockstep (independent
+  // counts on the scalar unit
+  const float nonpos = __builtin_amdgcn_ballot_w64(act && !(d > 0.0f)) != 0 ? 1.0f : 0.0f;
+  const float ddom = (float)__builtin_popcountll(__builtin_amdgcn_ballot_w64(act && d > r));
+  // reductions over the 64 lanes, the seven in lockstep (independent
   // instructions between each lane move and its use): rows of 16 by DPP,
-  // then across rows by permlane swaps.  Kinds: 0 min, 1 max, 2 sum.
-  float v[9] = {dmin, dmax, ninf, gers, dom, dsum, lsum, nonpos, ddom};
-  constexpr int kind[9] = {0, 1, 1, 0, 1, 2, 2, 1, 2};
+  // then across rows by lane swaps.  Kinds: 0 min, 1 max, 2 sum.
+  float v[7] = {dmin, dmax, ninf, gers, dom, dsum, lsum};
+  constexpr int kind[7] = {0, 1, 1, 0, 1, 2, 2};
   auto op = [&](int q, float x, float y) {
     return kind[q] == 0 ? __builtin_fminf(x, y) : kind[q] == 1 ? __builtin_fmaxf(x, y) : x + y;
   };
-  static_for<0, 9>([&](auto q) { v[q] = op(q, v[q], dppf<0xB1>(v[q])); });
-  static_for<0, 9>([&](auto q) { v[q] = op(q, v[q], dppf<0x4E>(v[q])); });
-  static_for<0, 9>([&](auto q) { v[q] = op(q, v[q], dppf<0x141>(v[q])); });
-  static_for<0, 9>([&](auto q) { v[q] = op(q, v[q], dppf<0x140>(v[q])); });
-  static_for<0, 9>([&](auto q) {
+  static_for<0, 7>([&](auto q) { v[q] = op(q, v[q], dppf<0xB1>(v[q])); });
+  static_for<0, 7>([&](auto q) { v[q] = op(q, v[q], dppf<0x4E>(v[q])); });
+  static_for<0, 7>([&](auto q) { v[q] = op(q, v[q], dppf<0x141>(v[q])); });
+  static_for<0, 7>([&](auto q) { v[q] = op(q, v[q], dppf<0x140>(v[q])); });
+  static_for<0, 7>([&](auto q) {
     const unsigned u = __builtin_bit_cast(unsigned, v[q]);
     const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
     v[q] = op(q, __builtin_bit_cast(float, (unsigned)a[0]), __builtin_bit_cast(float, (unsigned)a[1]));
   });
-  static_for<0, 9>([&](auto q) {
+  static_for<0, 7>([&](auto q) {
     const unsigned u = __builtin_bit_cast(unsigned, v[q]);
     const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
     v[q] = op(q, __builtin_bit_cast(float, (unsigned)a[0]), __builtin_bit_cast(float, (unsigned)a[1]));
   });
-  dmin = v[0], dmax = v[1], ninf = v[2], gers = v[3], dom = v[4], dsum = v[5], lsum = v[6], nonpos = v[7],
-  ddom = v[8];
+  dmin = v[0], dmax = v[1], ninf = v[2], gers = v[3], dom = v[4], dsum = v[5], lsum = v[6];
   const float fr = (float)anorm, rn = __builtin_amdgcn_rcpf((float)n);
   f[0] = (float)n * (1.0f / 64.0f);
   f[1] = flog10(dmin);
@@ -644,80 +676,47 @@ __device__ __forceinline__ void mx_features(const double (&ad)[nupper<NBLK>()][4
   f[11] = ddom * rn;
 }
 
-// W1 and W2 regrouped for the network below: element (o, k) of W (row o =
-// hidden unit o) at ((k / 4) * 64 + o) * 4 + k % 4, so that lane o's four
-// weights k .. k+3 are one 16-byte chunk and the 64 lanes read 1 KiB of
-// consecutive chunks (in the parameter layout each lane reads its own row,
-// and every load touches 64 lines).  W1t: 3 chunks per lane, then W2t: 16
-// (kTileW1, kTileW2, kTileFloats in router_layout.hpp; written by
-// k_router_tiles, router.hip).
-
-// The network on one wave (lane o = hidden unit o); returns the tier, the
-// first largest logit (a NaN logit gives tier 2).  hs: 64 floats of LDS.
-// The weights are read where they are used (W1 and W2 from the regrouped copy
-// Wt, column o of W3 from P), which keeps them out of the registers that hold
-// the matrix.  The sums run in the same order as k_router_forward's.
-__device__ __forceinline__ int mx_route(const float* __restrict__ P, const float* __restrict__ Wt,
-                                        const float (&f)[kRF], int lane, float* hs) {
-  float a1 = P[kOffB1 + lane];
-  const float4* w1 = reinterpret_cast<const float4*>(Wt + kTileW1) + lane;
-  static_for<0, kRF / 4>([&](auto c) {
-    const float4 w = w1[c * kRH];
-    const float wq[4] = {w.x, w.y, w.z, w.w};
-    static_for<0, 4>([&](auto e) {
-      constexpr int q = 4 * decltype(c)::value + decltype(e)::value;
-      a1 = __builtin_fmaf(wq[e], (f[q] - P[q]) * __builtin_amdgcn_rcpf(P[kRF + q]), a1);
-    });
-  });
-  hs[lane] = a1 > 0.0f ? a1 : 0.0f;
-  wave_sync();
-  float a2 = P[kOffB2 + lane];
-  const float4* w2 = reinterpret_cast<const float4*>(Wt + kTileW2) + lane;
-  static_for<0, kRH / 4>([&](auto q) {
-    const float4 w = w2[q * kRH];
-    const float4 h = *reinterpret_cast<const float4*>(hs + 4 * q);
-    a2 = __builtin_fmaf(w.x, h.x, a2);
-    a2 = __builtin_fmaf(w.y, h.y, a2);
-    a2 = __builtin_fmaf(w.z, h.z, a2);
-    a2 = __builtin_fmaf(w.w, h.w, a2);
-  });
-  const float h2 = a2 > 0.0f ? a2 : 0.0f;
-  float z[kRT];
-  static_for<0, kRT>([&](auto t) {
-    z[t] = group_allreduce<64>(P[kOffW3 + t * kRH + lane] * h2, [](float u, float v) { return u + v; }) + P[kOffB3 + t];
-  });
-  int t = 0;
-  t = z[1] > z[0] ? 1 : t;
-  t = z[2] > (t == 1 ? z[1] : z[0]) ? 2 : t;
-  if (!(z[0] == z[0] && z[1] == z[1] && z[2] == z[2])) t = 2;
-  return t;
-}
-
 // Routed posv: features and network in the wave that then solves the matrix
-// (A is read from HBM once): tier 0 / 1 factor in bf16 / fp32 and refine,
-// tier 2 goes straight to the float64 list (stat -1).  tier_out / feat_out
+// (A is read from HBM once): tier 1 factors in fp32 and refines here, tier 0
+// goes to the bfloat16 list (a.llist, solved by k_posv_mx_list: with both
+// refinement tiers inline the kernel would not fit its registers), tier 2 to
+// the float64 list (stat -1).  prep: the network
+// prepared for the matrix cores (router_net.hpp); tier_out / feat_out
 // (optional) receive the choice and the features.
 template <int NBLK>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void k_posv_mx_routed(
-    IrArgs a, const float* __restrict__ P, const float* __restrict__ Wt, int* __restrict__ tier_out,
-    float* __restrict__ feat_out) {
+    IrArgs a, const void* __restrict__ prep, int* __restrict__ tier_out, float* __restrict__ feat_out) {
   __shared__ __align__(16) double xs[16 * NBLK];
   __shared__ __align__(16) float fs[16];
-  __shared__ __align__(16) float hs[kRH];
+  __shared__ __align__(16) float dl[64];
+  __shared__ __align__(16) __bf16 hs[kNetLdsHalves];
   __shared__ v4f wl[nupper<NBLK>() * kWave];
-  __shared__ __align__(16) float ul[80 * NBLK];
   const int lane = threadIdx.x, g = lane >> 4, j = lane & 15;
   const int64_t m = a.m0 + blockIdx.x;
   double ad[nupper<NBLK>()][4];
   const double anorm = mx_load<NBLK>(a, m, g, j, ad);
   float f[kRF];
-  mx_features<NBLK>(ad, a.n, g, j, anorm, f, reinterpret_cast<float*>(wl), ul);
+  mx_features<NBLK>(ad, a.n, g, j, anorm, f, dl);
   if (feat_out && lane < kRF) {
     float v = f[0];
     static_for<1, kRF>([&](auto q) { v = (lane == q) ? f[q] : v; });
     feat_out[m * kRF + lane] = v;
   }
-  const int t = mx_route(P, Wt, f, lane, hs);
+  // the network's layer-1 operand: features 8 (l / 16) .. +7 in lane l
+  v8bf x;
+  static_for<0, 8>([&](auto e) {
+    constexpr int e0 = decltype(e)::value;
+    const float lo = f[e0], hi = e0 < 4 ? f[8 + e0] : 0.0f;
+    x[e0] = (__bf16)(g == 0 ? lo : g == 1 ? hi : 0.0f);
+  });
+  const v4fn z = net_logits(static_cast<const v8bf*>(prep),
+                            reinterpret_cast<const float*>(static_cast<const char*>(prep) + kPrepBiasByte), x, hs,
+                            lane);
+  // every column holds this matrix: lane 0 has its logits
+  const float z0 = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, (float)z[0])));
+  const float z1 = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, (float)z[1])));
+  const float z2 = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, (float)z[2])));
+  const int t = net_tier(z0, z1, z2);
   if (tier_out && lane == 0) tier_out[m] = t;
   if (t == 2) {
     if (lane == 0) {
@@ -726,10 +725,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     }
     return;
   }
-  if (t == 0)
-    mx_solve<NBLK, Low::BF16>(a, m, g, j, ad, anorm, xs, fs, wl);
-  else
-    mx_solve<NBLK, Low::F32>(a, m, g, j, ad, anorm, xs, fs, wl);
+  if (t == 0) {
+    if (lane == 0) a.llist[atomicAdd(a.lcount, 1)] = (int)m;
+    return;
+  }
+  mx_solve<NBLK, Low::F32>(a, m, g, j, ad, anorm, xs, fs, wl);
 }
 
 }  // namespace tsdx_hip

@@ -19,6 +19,11 @@ struct IrArgs {
   int* stat;    // (batch) or null: corrections used, or -1 when sent to the float64 solve
   int* flist;   // (batch) matrices handed to the float64 solve
   int* fcount;  // (1) their number
+  // The routed solve's list of matrices for the bfloat16 tier (filled by the
+  // routed kernel, solved by a list launch of the refinement kernel): list
+  // llist[0 .. *lcount).
+  int* llist = nullptr;
+  int* lcount = nullptr;
   // First matrix of this launch: workgroup w of the one-wave-per-matrix
   // kernels solves matrix m0 + w.  Large batches are split into launches of
   // at most kIrChunk workgroups (a grid must stay below 2^32 work items).

@@ -40,12 +40,18 @@ static void launch_mx(const IrArgs& a0, int low, hipStream_t s) {
 // hands its own failures to the plain posv in the same wave) unless `plain`
 // asks for the plain posv throughout; the plain posv for smaller sizes, whose
 // buckets put several matrices in a wave.
+// full: the list holds every matrix (a float64 factor was asked for); else it
+// holds the refinement's failures and the router's float64 choices, usually
+// few, so a smaller grid strides over it (workgroups of an empty list still
+// cost their dispatch).
+constexpr unsigned kGatherGridShort = 1024;
 template <int NB>
-static hipError_t gather(const IrArgs& a, hipStream_t s, bool plain) {
+static hipError_t gather(const IrArgs& a, hipStream_t s, bool plain, bool full) {
   if constexpr (NB == 64) {
     if (!plain) {
       // two waves per SIMD fit this kernel: 4096 workgroups fill the GPU twice
-      const dim3 gg((unsigned)(a.batch < 4096 ? a.batch : 4096));
+      const unsigned cap = full ? 4096u : kGatherGridShort;
+      const dim3 gg((unsigned)(a.batch < (int64_t)cap ? a.batch : cap));
       if (a.nrhs <= 1)
         hipLaunchKernelGGL((k_posv_mx64<4, 1>), gg, dim3(kWave), 0, s, a);
       else
@@ -54,7 +60,8 @@ static hipError_t gather(const IrArgs& a, hipStream_t s, bool plain) {
     }
   }
   const dim3 g = grid_for<NB>(a.batch);
-  const dim3 gg(g.x < 16384u ? g.x : 16384u);
+  const unsigned cap = full ? 16384u : kGatherGridShort;
+  const dim3 gg(g.x < cap ? g.x : cap);
   if (a.nrhs <= 1)
     hipLaunchKernelGGL((k_posv_gather<NB, 1>), gg, block_for<NB>(), 0, s, a);
   else
@@ -63,11 +70,11 @@ static hipError_t gather(const IrArgs& a, hipStream_t s, bool plain) {
 }
 
 hipError_t launch_posv_gather(const IrArgs& a, hipStream_t s) {
-  if (a.n <= 4) return gather<4>(a, s, false);
-  if (a.n <= 8) return gather<8>(a, s, false);
-  if (a.n <= 16) return gather<16>(a, s, false);
-  if (a.n <= 32) return gather<32>(a, s, false);
-  return gather<64>(a, s, false);
+  if (a.n <= 4) return gather<4>(a, s, false, false);
+  if (a.n <= 8) return gather<8>(a, s, false, false);
+  if (a.n <= 16) return gather<16>(a, s, false, false);
+  if (a.n <= 32) return gather<32>(a, s, false, false);
+  return gather<64>(a, s, false, false);
 }
 
 template <int NB>
@@ -78,7 +85,7 @@ static hipError_t launch_ir(const IrArgs& a, int low, hipStream_t s) {
   } else {  // float64 for every matrix (4 plain, 5 the fallback tier): the list is 0..batch-1
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((a.batch + 255) / 256)), dim3(256), 0, s, a.flist, a.fcount, a.stat, a.batch);
   }
-  return gather<NB>(a, s, low == 4);
+  return gather<NB>(a, s, low == 4, low == 4 || low == 5);
 }
 
 hipError_t launch_posv_ir(const IrArgs& a, int low, hipStream_t s) {

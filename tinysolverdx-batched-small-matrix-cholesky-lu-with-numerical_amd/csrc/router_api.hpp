@@ -20,8 +20,12 @@ hipError_t launch_router_adam(float* P, const float* grad, float* m, float* v, i
 // network and solves in the chosen tier; then the float64 solve of tier-2
 // matrices and of every fallback.  tier / feat (optional) receive the choice
 // and the features.
-// Wt: kTileFloats floats of device memory (16-byte aligned) for the regrouped
-// first and second layer weights.
-hipError_t launch_posv_routed(const IrArgs& a, const float* P, float* Wt, int* tier, float* feat, hipStream_t s);
+// prep: kRouterPrepBytes of device memory (256-byte aligned) for the network
+// prepared for the matrix cores (router_layout.hpp).
+hipError_t launch_posv_routed(const IrArgs& a, const float* P, void* prep, int* tier, float* feat, hipStream_t s);
+// Tiers (and, if logits is not null, logits) of `count` feature vectors from
+// the matrix-core network the routed posv runs.
+hipError_t launch_router_infer(const float* P, const float* feat, int64_t count, int* tier, float* logits, void* prep,
+                               hipStream_t s);
 
 }  // namespace tsdx_hip

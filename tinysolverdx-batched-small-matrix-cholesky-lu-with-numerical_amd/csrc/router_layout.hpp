@@ -14,9 +14,19 @@ constexpr int kRF = 12, kRH = 64, kRT = 3;
 constexpr int kOffW1 = 24, kOffB1 = kOffW1 + kRH * kRF, kOffW2 = kOffB1 + kRH, kOffB2 = kOffW2 + kRH * kRH,
               kOffW3 = kOffB2 + kRH, kOffB3 = kOffW3 + kRT * kRH, kRouterParams = kOffB3 + kRT;
 static_assert(kRouterParams == 5211, "parameter layout");
-// The regrouped W1 and W2 the in-wave network reads (k_router_tiles).
-constexpr int kTileW1 = 0, kTileW2 = kRH * kRF, kTileFloats = kTileW2 + kRH * kRH;
-constexpr size_t kRouterTileBytes = kTileFloats * sizeof(float);
+// The network prepared for the matrix cores (k_router_prep, router_net.hpp):
+// kPrepFrags bfloat16 A-operand fragments of v_mfma_f32_16x16x32_bf16, 16 bytes
+// per lane, fragment f of lane l at byte (64 f + l) * 16:
+//   f = t (0..3):            W1'[16t + l % 16][8 (l / 16) + e]   (inputs >= 12 zero)
+//   f = 4 + 2t + c (c = 0,1): W2[16t + l % 16][32c + 8 (l / 16) + e]
+//   f = 12 + c:              W3[l % 16][32c + 8 (l / 16) + e]     (outputs >= 3 zero)
+// then float biases at kPrepBiasByte: b1' (64), b2 (64 from kPrepB2), b3
+// (padded to 16 from kPrepB3), with W1' = W1 / scale and b1' = b1 - W1' mean
+// (the feature normalisation folded into layer 1).
+constexpr int kPrepFrags = 14;
+constexpr size_t kPrepBiasByte = (size_t)kPrepFrags * 64 * 16;
+constexpr int kPrepB2 = 64, kPrepB3 = 128, kPrepBiasFloats = 144;
+constexpr size_t kRouterPrepBytes = kPrepBiasByte + kPrepBiasFloats * sizeof(float);
 
 // log10 with non-positive (and tiny) arguments clamped to -30: log2 on the
 // transcendental unit (arguments past the clamp are normal numbers, which it

@@ -47,6 +47,8 @@ SIGNATURES["tsdx_hip_ragged_plan_report"] = [_vp]
 SIGNATURES["tsdx_hip_ragged_plan_destroy"] = [_vp]
 SIGNATURES["tsdx_hip_router_features_f64"] = [_vp, _i, _i64, _vp, _vp]
 SIGNATURES["tsdx_hip_router_forward"] = [_vp, _vp, _i64, _vp, _vp]
+SIGNATURES["tsdx_hip_router_infer_workspace_size"] = []
+SIGNATURES["tsdx_hip_router_infer"] = [_vp, _vp, _i64, _vp, _vp, _vp, C.c_size_t, _vp]
 SIGNATURES["tsdx_hip_router_grad_workspace_size"] = [_i64]
 SIGNATURES["tsdx_hip_router_grad"] = [_vp, _vp, _vp, _i64, _vp, _vp, _vp, C.c_size_t, _vp]
 SIGNATURES["tsdx_hip_router_adam"] = [_vp, _vp, _vp, _vp, _i, C.c_float, _vp]
@@ -58,7 +60,7 @@ SIGNATURES["tsdx_hip_measure_peak_gflops"] = [_i, C.POINTER(_d)]
 SIGNATURES["tsdx_hip_measure_mfma_tflops"] = [_i, C.POINTER(_d)]
 _RES = {"tsdx_hip_last_error": C.c_char_p, "tsdx_hip_posv_ir_workspace_size": C.c_size_t,
         "tsdx_hip_ragged_plan_report": C.c_char_p, "tsdx_hip_router_grad_workspace_size": C.c_size_t,
-        "tsdx_hip_posv_routed_workspace_size": C.c_size_t}
+        "tsdx_hip_posv_routed_workspace_size": C.c_size_t, "tsdx_hip_router_infer_workspace_size": C.c_size_t}
 
 KIND = {"cholesky": 0, "lu": 1}
 

@@ -148,30 +148,44 @@ def choose_direct(r, sizes=(16, 32, 64), count=65536, kmax=1e12, seed=20261210, 
     return sorted(direct), times
 
 
-def training_set(count_per_n=20000, sizes=(16, 32, 64), kmax=1e12, seed=20261207, device="cuda"):
-    """Features and labels of SPD matrices with kappa log-uniform on [1, kmax]
-    (one block per size), generated on the host from `seed`."""
+# Training mix of the shipped router: per size, (count, kappa_min, kappa_max)
+# blocks of SPD matrices built like the posv workloads.  Every size covers
+# kappa up to 1e12 (the routed-posv workload); n = 64 adds the kappa range of
+# the n = 64 mixed-precision posv workload (1e1..1e6), so the prior the network
+# learns matches the mix it is used on there.
+TRAINING_MIX = {16: [(20000, 1.0, 1e12)], 32: [(20000, 1.0, 1e12)], 64: [(20000, 1.0, 1e12), (20000, 1e1, 1e6)]}
+
+
+def training_set(mix=None, seed=20261207, device="cuda"):
+    """Features and labels of SPD matrices built like the posv workloads
+    (matrices.spd_device: Q diag(lambda) Q^T with Q Haar, lambda_0 = 1,
+    lambda_{n-1} = 1/kappa and the other eigenvalues log-uniform in between,
+    kappa log-uniform per block), for the blocks of `mix` ({n: [(count,
+    kappa_min, kappa_max), ...]}, default TRAINING_MIX), seeded from `seed`."""
     import torch
     from . import matrices
-    rng = np.random.default_rng(seed)
+    mix = TRAINING_MIX if mix is None else mix
     feats, labels = [], []
     costs = {}
-    for n in sizes:
-        kappa = 10.0 ** rng.uniform(0.0, np.log10(kmax), count_per_n)
-        A = torch.from_numpy(matrices.spd_kappas(rng, kappa, n)).to(device)
-        B = torch.from_numpy(rng.standard_normal((count_per_n, n, 1))).to(device)
+    for n, blocks in mix.items():
         costs[n] = measure_costs(n)
-        labels.append(make_labels(A, B, costs[n]))
-        feats.append(features(A))
-        del A, B
+        for b, (count, kmin, kmax) in enumerate(blocks):
+            s = seed + 101 * n + b
+            A, _ = matrices.spd_device(n, count, kmin, kmax, s, device=device)
+            B = torch.randn((count, n, 1), generator=torch.Generator(device=device).manual_seed(s + 1),
+                            device=device, dtype=torch.float64)
+            labels.append(make_labels(A, B, costs[n]))
+            feats.append(features(A))
+            del A, B
     training_set.costs = costs
     return torch.cat(feats), torch.cat(labels)
 
 
-def train_default(path=DEFAULT_PARAMS, count_per_n=20000, epochs=60, seed=20261207):
-    """Trains the router shipped with the package (kappa up to 1e12, n in
-    {16, 32, 64}) with the library's kernels and writes its parameters."""
-    feat, label = training_set(count_per_n, seed=seed)
+def train_default(path=DEFAULT_PARAMS, mix=None, epochs=60, seed=20261207):
+    """Trains the router shipped with the package (TRAINING_MIX: kappa up to
+    1e12 for n in {16, 32, 64}, plus the 1e1..1e6 range at n = 64) with the
+    library's kernels and writes its parameters."""
+    feat, label = training_set(mix, seed=seed)
     r = Router(seed=seed)
     hist = r.fit(feat, label, epochs=epochs, batch=4096, lr=3e-3, seed=seed)
     acc = float((r.predict(feat) == label).float().mean().item())
@@ -241,8 +255,27 @@ class Router:
         import torch
         return c.tensor(self.params, "params", torch.float32, (PARAMS,))
 
+    def infer(self, feat, work=None):
+        """(tiers, logits) from the network as the routed posv runs it: on the
+        matrix cores with bfloat16 weights and activations
+        (tsdx_hip_router_infer), so the tiers are the routed solve's choices
+        for these features; logits within bfloat16 rounding of forward()."""
+        import torch
+        c = self._features(feat)
+        tier = torch.empty(c.batch, dtype=torch.int32, device=feat.device)
+        logits = torch.empty((c.batch, TIERS), dtype=torch.float32, device=feat.device)
+        if c.batch:
+            need = _lib.lib().tsdx_hip_router_infer_workspace_size()
+            if work is None:
+                work = torch.empty(need, dtype=torch.uint8, device=feat.device)
+            w, wn = c.work(work, need)
+            _lib.call("tsdx_hip_router_infer", self._params(c), c.mat(), c.batch, tier.data_ptr(), logits.data_ptr(),
+                      w, wn, c.stream())
+        return tier, logits
+
     def predict(self, feat):
-        return self.forward(feat).argmax(dim=1).to(dtype=__import__("torch").int32)
+        """Tiers the routed posv picks for these features."""
+        return self.infer(feat)[0]
 
     # ----------------------------------------------------------------- training
     def grad(self, feat, label, work=None):
@@ -368,8 +401,7 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="train the default precision router on the GPU")
     ap.add_argument("--train", action="store_true")
     ap.add_argument("--out", default=str(DEFAULT_PARAMS))
-    ap.add_argument("--count-per-n", type=int, default=20000)
     ap.add_argument("--epochs", type=int, default=60)
     args = ap.parse_args()
     if args.train:
-        print(json.dumps(train_default(args.out, args.count_per_n, args.epochs)), flush=True)
+        print(json.dumps(train_default(args.out, epochs=args.epochs)), flush=True)
