@@ -141,6 +141,30 @@ int tsdx_hip_posv_ir_noalloc_f64(const double *A, int n, int64_t batch, const do
                                  int factor, int max_iter, double tol, int *info, int *stat, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Mixed-precision gesv with iterative refinement (float64 A, B, X).  No
+ * reference counterpart: the reference has only the plain gesv_batched
+ * (batched.hpp:276-280, solve by lu_solve_rm, kernels.hpp:204-227).  Each
+ * matrix is factored with partial pivoting in the low precision `factor`,
+ * each right-hand side refined with float64 residuals until
+ * ||b - Ax|| <= tol/2 (||A||_F ||x|| + ||b||) (the test of
+ * tsdx_hip_posv_ir_f64), at most max_iter corrections; a matrix whose
+ * low-precision factorization meets a zero pivot, or whose refinement stalls
+ * or does not reach tol, is solved again by the plain float64 gesv, which
+ * also gives its info (as tsdx_hip_gesv_solve_f64).  Converged matrices get
+ * info 0.  Only X, info and stat are written.
+ *   factor: TSDX_HIP_FACTOR_FP32 / _BF16 / _FP16 (low precision + refinement),
+ *           _FP64 (plain float64 gesv for every matrix), _AUTO (FP32);
+ *           _FP64_BLOCK is rejected (there is no block float64 LU tier).
+ *   stat:   (batch) or NULL: corrections used (largest over the right-hand
+ *           sides), or -1 for a matrix solved by the float64 gesv.
+ *   work:   device workspace of tsdx_hip_gesv_ir_workspace_size(batch) bytes
+ *           (the float64 list); calls sharing one workspace must not overlap.
+ * ------------------------------------------------------------------------ */
+size_t tsdx_hip_gesv_ir_workspace_size(int64_t batch);
+int tsdx_hip_gesv_ir_f64(const double *A, int n, int64_t batch, const double *B, double *X, int nrhs, int factor,
+                         int max_iter, double tol, int *info, int *stat, void *work, size_t work_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Learned precision router for posv (no reference counterpart).  A small
  * network picks, per matrix, the cheapest tier expected to reach the
  * tolerance: 0 = bf16 factor + float64 refinement, 1 = fp32 factor + float64

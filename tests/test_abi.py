@@ -72,6 +72,15 @@ def test_abi_argument_errors_without_device():
     assert L.tsdx_hip_posv_ir_f64(p, 8, 1, p, p, 1, 1, 10, -1.0, None, None, None, 0, None) == _lib.EINVAL
     assert L.tsdx_hip_posv_ir_f64(p, 8, 1, p, p, 1, 1, 10, 1e-14, None, None, None, 0, None) == _lib.EINVAL
     assert "workspace" in _lib.last_error()
+    assert L.tsdx_hip_gesv_ir_workspace_size(1000) >= 4 * 1000
+    assert L.tsdx_hip_gesv_ir_f64(p, 65, 1, p, p, 1, 1, 10, 1e-14, None, None, None, 0, None) == _lib.EINVAL
+    assert L.tsdx_hip_gesv_ir_f64(p, 8, 1, p, p, 1, 5, 10, 1e-14, None, None, None, 0, None) == _lib.EINVAL
+    assert "factor" in _lib.last_error()
+    assert L.tsdx_hip_gesv_ir_f64(p, 8, 1, p, p, 1, 1, 1001, 1e-14, None, None, None, 0, None) == _lib.EINVAL
+    assert L.tsdx_hip_gesv_ir_f64(p, 8, 1, p, p, 1, 1, 10, 0.0, None, None, None, 0, None) == _lib.EINVAL
+    assert L.tsdx_hip_gesv_ir_f64(p, 8, 1, p, p, 1, 1, 10, 1e-14, None, None, None, 0, None) == _lib.EINVAL
+    assert "workspace" in _lib.last_error()
+    assert L.tsdx_hip_gesv_ir_f64(None, 8, 0, None, None, 1, 1, 10, 1e-14, None, None, None, 0, None) == _lib.OK
 
 
 def test_python_error_types(tsdx):

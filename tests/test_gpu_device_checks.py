@@ -52,6 +52,16 @@ def test_posv_ir_and_router_shapes():
         device.posv_ir(A, B, X, work=_t((8,), "uint8"))                          # workspace too small
     with pytest.raises(ValueError):
         device.posv_ir(A, B, X, factor="fp8")
+    with pytest.raises(TypeError):
+        device.gesv_ir(_t((4, 64, 64), "float32"), B, X)
+    with pytest.raises(ValueError):
+        device.gesv_ir(A, B, _t((4, 64, 2)))
+    with pytest.raises(ValueError):
+        device.gesv_ir(A, B, X, info=_t((5,), "int32"))
+    with pytest.raises(ValueError):
+        device.gesv_ir(A, B, X, work=_t((8,), "uint8"))                          # workspace too small
+    with pytest.raises(ValueError):
+        device.gesv_ir(A, B, X, factor="fp64_block")                              # no block LU tier
     r = router.Router(device="cuda")
     with pytest.raises(ValueError):
         r.posv(A, B, X, tier=_t((5,), "int32"))

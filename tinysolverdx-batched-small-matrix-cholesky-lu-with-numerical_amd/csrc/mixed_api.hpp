@@ -37,4 +37,11 @@ hipError_t launch_posv_ir(const IrArgs& a, int low, hipStream_t s);
 // float64 posv of the matrices on a.flist[0 .. *a.fcount).
 hipError_t launch_posv_gather(const IrArgs& a, hipStream_t s);
 
+// Mixed-precision gesv of bucket NB (gesv_ir.hpp): refinement in low (1 float,
+// 2 bfloat16, 3 half), then the float64 gesv of the list a.flist, whose count
+// *a.fcount the caller has cleared.  ztile: zero-pivot rule of the float64
+// gesv (divide by one, n <= 24).
+template <int NB>
+hipError_t launch_gesv_ir_nb(const IrArgs& a, int low, bool ztile, hipStream_t s);
+
 }  // namespace tsdx_hip

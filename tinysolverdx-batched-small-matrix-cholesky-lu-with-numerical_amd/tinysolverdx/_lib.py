@@ -40,6 +40,8 @@ for _t in ("f32", "f64"):
 SIGNATURES["tsdx_hip_posv_ir_workspace_size"] = [_i64]
 SIGNATURES["tsdx_hip_posv_ir_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, C.c_size_t, _vp]
 SIGNATURES["tsdx_hip_posv_ir_noalloc_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp]
+SIGNATURES["tsdx_hip_gesv_ir_workspace_size"] = [_i64]
+SIGNATURES["tsdx_hip_gesv_ir_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, C.c_size_t, _vp]
 for _t in ("f32", "f64"):
     SIGNATURES[f"tsdx_hip_ragged_plan_create_{_t}"] = [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i, C.POINTER(_vp), _vp]
 SIGNATURES["tsdx_hip_ragged_run"] = [_vp, _vp]
@@ -59,6 +61,7 @@ SIGNATURES["tsdx_hip_measure_bandwidth_gbs"] = [_i, C.POINTER(_d)]
 SIGNATURES["tsdx_hip_measure_peak_gflops"] = [_i, C.POINTER(_d)]
 SIGNATURES["tsdx_hip_measure_mfma_tflops"] = [_i, C.POINTER(_d)]
 _RES = {"tsdx_hip_last_error": C.c_char_p, "tsdx_hip_posv_ir_workspace_size": C.c_size_t,
+        "tsdx_hip_gesv_ir_workspace_size": C.c_size_t,
         "tsdx_hip_ragged_plan_report": C.c_char_p, "tsdx_hip_router_grad_workspace_size": C.c_size_t,
         "tsdx_hip_posv_routed_workspace_size": C.c_size_t, "tsdx_hip_router_infer_workspace_size": C.c_size_t}
 

@@ -208,6 +208,34 @@ def posv_ir(A, B, X, factor="fp32", max_iter=10, tol=1e-14, info=None, stat=None
     return X, info, stat
 
 
+def gesv_ir_workspace(batch, device="cuda"):
+    """A device workspace for gesv_ir (uint8 tensor)."""
+    nbytes = _lib.lib().tsdx_hip_gesv_ir_workspace_size(batch)
+    return _torch().empty(nbytes, dtype=_torch().uint8, device=device)
+
+
+def gesv_ir(A, B, X, factor="fp32", max_iter=10, tol=1e-14, info=None, stat=None, work=None):
+    """Mixed-precision gesv with float64 iterative refinement (float64 A, B, X).
+
+    factor: "fp32", "bf16", "fp16" (partial-pivoting LU in that precision +
+    refinement), "fp64" (plain float64 gesv for every matrix) or "auto".  A
+    matrix whose low-precision LU meets a zero pivot, or whose refinement does
+    not reach tol, is solved again by the float64 gesv, which also gives its
+    info; converged matrices get info 0.  stat (int32) gets the corrections
+    used per matrix, -1 for the float64 solve.  A and B are only read."""
+    torch = _torch()
+    c = Call(A, dtypes=(torch.float64,))
+    b, nrhs = c.rhs(B)
+    if factor not in _lib.FACTOR or factor == "fp64_block":
+        raise ValueError("factor must be one of auto, fp32, bf16, fp16, fp64")
+    if work is None:
+        work = gesv_ir_workspace(c.batch, A.device)
+    w, wn = c.work(work, _lib.lib().tsdx_hip_gesv_ir_workspace_size(c.batch))
+    _lib.call("tsdx_hip_gesv_ir_f64", c.mat(), c.n, c.batch, b, c.like(X, B), nrhs, _lib.FACTOR[factor],
+              int(max_iter), float(tol), c.ints(info, "info"), c.ints(stat, "stat"), w, wn, c.stream())
+    return X, info, stat
+
+
 class RaggedPlan:
     """Matrices of different sizes and kinds factored by one graph launch.
 
