@@ -300,6 +300,54 @@ int tsdx_hip_posv_batched_f64(double *A, int n, int64_t batch, double *B, int nr
 int tsdx_hip_gesv_batched_f32(float *A, int n, int64_t batch, float *B, int nrhs, int *piv, int *info);
 int tsdx_hip_gesv_batched_f64(double *A, int n, int64_t batch, double *B, int nrhs, int *piv, int *info);
 
+/* ---------------------------------------------------------------------------
+ * Emulated fp16 / bfloat16: the reference's runtime-precision calls with
+ * Precision::FP16 / BF16 (runtime.cpp:72-140; Emu<F> arithmetic,
+ * precision.hpp:71-97; Auto resolves to the scalar kernels cholesky_rm, lu_rm,
+ * cholesky_solve_rm, lu_solve_rm, kernels.hpp:18-37, 91-119, 184-227).  The
+ * caller's float32 / float64 array is read with every entry rounded to the
+ * format (float64 through float first), each operation is computed in float
+ * and rounded to the format, and the results are written back in the caller's
+ * storage type: factors (a failed Cholesky leaves the entries the reference
+ * leaves untouched as the rounded input), pivots, info and solutions are
+ * bit-identical to the reference's.
+ *   format: TSDX_HIP_EMU_FP16 or TSDX_HIP_EMU_BF16 (TSDX_HIP_EINVAL otherwise).
+ * posv / gesv are the factor followed by the solve (posv_batched,
+ * gesv_batched, batched.hpp:271-280).
+ * ------------------------------------------------------------------------ */
+#define TSDX_HIP_EMU_FP16 1
+#define TSDX_HIP_EMU_BF16 2
+
+/* stream-ordered, device pointers */
+int tsdx_hip_potrf_emu_f32(float *A, int n, int64_t batch, int *info, int format, void *stream);
+int tsdx_hip_potrf_emu_f64(double *A, int n, int64_t batch, int *info, int format, void *stream);
+int tsdx_hip_getrf_emu_f32(float *A, int n, int64_t batch, int *piv, int *info, int format, void *stream);
+int tsdx_hip_getrf_emu_f64(double *A, int n, int64_t batch, int *piv, int *info, int format, void *stream);
+int tsdx_hip_potrs_emu_f32(const float *L, int n, int64_t batch, float *B, int nrhs, int format, void *stream);
+int tsdx_hip_potrs_emu_f64(const double *L, int n, int64_t batch, double *B, int nrhs, int format, void *stream);
+int tsdx_hip_getrs_emu_f32(const float *LU, const int *piv, int n, int64_t batch, float *B, int nrhs, int format,
+                           void *stream);
+int tsdx_hip_getrs_emu_f64(const double *LU, const int *piv, int n, int64_t batch, double *B, int nrhs, int format,
+                           void *stream);
+
+/* synchronous, host pointers */
+int tsdx_hip_cholesky_emu_batched_f32(float *A, int n, int64_t batch, int *info, int format);
+int tsdx_hip_cholesky_emu_batched_f64(double *A, int n, int64_t batch, int *info, int format);
+int tsdx_hip_lu_emu_batched_f32(float *A, int n, int64_t batch, int *piv, int *info, int format);
+int tsdx_hip_lu_emu_batched_f64(double *A, int n, int64_t batch, int *piv, int *info, int format);
+int tsdx_hip_cholesky_solve_emu_batched_f32(const float *L, int n, int64_t batch, float *B, int nrhs, int format);
+int tsdx_hip_cholesky_solve_emu_batched_f64(const double *L, int n, int64_t batch, double *B, int nrhs, int format);
+int tsdx_hip_lu_solve_emu_batched_f32(const float *LU, const int *piv, int n, int64_t batch, float *B, int nrhs,
+                                      int format);
+int tsdx_hip_lu_solve_emu_batched_f64(const double *LU, const int *piv, int n, int64_t batch, double *B, int nrhs,
+                                      int format);
+int tsdx_hip_posv_emu_batched_f32(float *A, int n, int64_t batch, float *B, int nrhs, int *info, int format);
+int tsdx_hip_posv_emu_batched_f64(double *A, int n, int64_t batch, double *B, int nrhs, int *info, int format);
+int tsdx_hip_gesv_emu_batched_f32(float *A, int n, int64_t batch, float *B, int nrhs, int *piv, int *info,
+                                  int format);
+int tsdx_hip_gesv_emu_batched_f64(double *A, int n, int64_t batch, double *B, int nrhs, int *piv, int *info,
+                                  int format);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,17 @@ class Oracle:
         self._sig("unit_roundoff", [ci], C.c_double)
         self._sig("set_num_threads", [ci], None)
         self._sig("num_threads", [], ci)
+        for sfx in ("f32", "f64"):
+            if kind == "port":
+                self._sig(f"emu_cholesky_{sfx}", [ci, vp, ci, i64, vp])
+                self._sig(f"emu_lu_{sfx}", [ci, vp, ci, i64, vp, vp])
+                self._sig(f"emu_chol_solve_{sfx}", [ci, vp, ci, i64, vp, ci])
+                self._sig(f"emu_lu_solve_{sfx}", [ci, vp, vp, ci, i64, vp, ci])
+            else:
+                self._sig(f"cholesky_{sfx}", [ci, vp, ci, i64, vp, ci])
+                self._sig(f"lu_{sfx}", [ci, vp, ci, i64, vp, vp, ci])
+                self._sig(f"cholesky_solve_{sfx}", [ci, vp, ci, i64, vp, ci])
+                self._sig(f"lu_solve_{sfx}", [ci, vp, vp, ci, i64, vp, ci])
         if kind == "port":
             self._sig("round_to_f64", [ci, vp, i64], None)
             self._sig("chol_rm_f64", [vp, ci])
@@ -132,6 +143,52 @@ class Oracle:
         info = np.zeros(A.shape[0], np.int32)
         self._check(self._f(f"gesv_batched_{_sfx(A.dtype)}")(_p(A), A.shape[-1], A.shape[0], _p(B), B.shape[-1], _p(piv), _p(info), BACKENDS[backend]))
         return A, B, piv, info
+
+    # ---- runtime precision (runtime.cpp:92-140): emulated fp16 / bf16 ----
+    # The port restates only the emulated formats; the reference takes all four.
+    def _emu(self, port_name, ref_name, prec, *args):
+        if self.kind == "port":
+            if prec not in ("fp16", "bf16"):
+                raise ValueError("the port restates the emulated precisions fp16 / bf16 only")
+            return self._check(self._f(port_name)(PRECISIONS[prec], *args))
+        return self._check(self._f(ref_name)(PRECISIONS[prec], *args))
+
+    def cholesky_p(self, A, prec):
+        """tsdx::cholesky(precision, ...) on a copy of A (float32 or float64 storage)."""
+        A = np.array(A, copy=True, order="C")
+        batch, n = A.shape[0], A.shape[-1]
+        info = np.zeros(batch, np.int32)
+        extra = () if self.kind == "port" else (BACKENDS["auto"],)
+        sfx = _sfx(A.dtype)
+        self._emu(f"emu_cholesky_{sfx}", f"cholesky_{sfx}", prec, _p(A), n, batch, _p(info), *extra)
+        return A, info
+
+    def lu_p(self, A, prec):
+        A = np.array(A, copy=True, order="C")
+        batch, n = A.shape[0], A.shape[-1]
+        piv = np.zeros((batch, n), np.int32)
+        info = np.zeros(batch, np.int32)
+        extra = () if self.kind == "port" else (BACKENDS["auto"],)
+        sfx = _sfx(A.dtype)
+        self._emu(f"emu_lu_{sfx}", f"lu_{sfx}", prec, _p(A), n, batch, _p(piv), _p(info), *extra)
+        return A, piv, info
+
+    def cholesky_solve_p(self, L, B, prec):
+        L = np.ascontiguousarray(L)
+        B = np.array(B, copy=True, order="C", dtype=L.dtype)
+        sfx = _sfx(L.dtype)
+        self._emu(f"emu_chol_solve_{sfx}", f"cholesky_solve_{sfx}", prec, _p(L), L.shape[-1], L.shape[0], _p(B),
+                  B.shape[-1])
+        return B
+
+    def lu_solve_p(self, LU, piv, B, prec):
+        LU = np.ascontiguousarray(LU)
+        piv = np.ascontiguousarray(piv, dtype=np.int32)
+        B = np.array(B, copy=True, order="C", dtype=LU.dtype)
+        sfx = _sfx(LU.dtype)
+        self._emu(f"emu_lu_solve_{sfx}", f"lu_solve_{sfx}", prec, _p(LU), _p(piv), LU.shape[-1], LU.shape[0],
+                  _p(B), B.shape[-1])
+        return B
 
     # ---- precision ----
     def round_fp16(self, x):

@@ -57,6 +57,12 @@ tsdx::Precision prec(int p) {
   int ref_lu_##SUF(int p, T* A, int n, long long batch, int* piv, int* info, int b) {                     \
     return guard([&] { tsdx::lu<T>(prec(p), A, n, batch, piv, info, be(b)); });                            \
   }                                                                                                         \
+  int ref_cholesky_solve_##SUF(int p, const T* L, int n, long long batch, T* B, int nrhs) {               \
+    return guard([&] { tsdx::cholesky_solve<T>(prec(p), L, n, batch, B, nrhs); });                         \
+  }                                                                                                         \
+  int ref_lu_solve_##SUF(int p, const T* LU, const int* piv, int n, long long batch, T* B, int nrhs) {    \
+    return guard([&] { tsdx::lu_solve<T>(prec(p), LU, piv, n, batch, B, nrhs); });                          \
+  }                                                                                                         \
   int ref_round_to_##SUF(int p, T* x, long long count) {                                                  \
     return guard([&] { tsdx::round_to<T>(prec(p), x, count); });                                            \
   }

@@ -10,6 +10,8 @@
  *   include/tinysolverdx/kernels.hpp:18-227   per-matrix and tile kernels
  *   include/tinysolverdx/batched.hpp:27-280   batch drivers, Auto rules, padding
  *   include/tinysolverdx/precision.hpp:41-65, 118-126  fp16/bf16 rounding, u
+ *   include/tinysolverdx/precision.hpp:71-97  Emu<F> (emulated fp16 / bf16 arithmetic)
+ *   src/runtime.cpp:72-140                     runtime-precision calls (Emu copies)
  *   src/runtime.cpp:143-150                    round_to
  *   src/runtime.cpp:162-260                    long-double error metrics
  *   src/runtime.cpp:15-58                      thread count (TSDX_NUM_THREADS)
@@ -162,6 +164,183 @@ void orc_round_to_f32(int p, float *x, int64_t count) {
   if (p == 1) return;
   for (int64_t i = 0; i < count; ++i) x[i] = p == 0 ? x[i] : p == 2 ? orc_round_fp16(x[i]) : orc_round_bf16(x[i]);
 }
+
+/* ------------------------------------------- emulated fp16 / bfloat16 */
+/* precision.hpp:71-97 (Emu<F>): a value is a float representable in the
+ * format; each of + - * / and sqrt is computed in float and its result rounded
+ * to the format (no fused multiply-add: `x -= y*z` is two rounded operations).
+ * runtime.cpp:72-140 (runtime-precision calls): the caller's array is copied
+ * into Emu values (double storage through float first), the batched kernel
+ * runs (Auto resolves to the scalar kernels for these types, batched.hpp:46-50),
+ * and the float values are copied back, every entry (untouched ones come back
+ * rounded).  p: 2 fp16, 3 bf16. */
+typedef float (*orc_rnd)(float);
+
+/* cholesky_rm with Emu operations (kernels.hpp:18-37): stops at the first
+ * failing pivot, leaving the rest of the matrix as it is. */
+static int emu_chol_rm(float *a, int n, orc_rnd R) {
+  for (int i = 0; i < n; ++i) {
+    float *ri = a + (size_t)i * n;
+    for (int j = 0; j <= i; ++j) {
+      const float *rj = a + (size_t)j * n;
+      float s = ri[j];
+      for (int k = 0; k < j; ++k) s = R(s - R(ri[k] * rj[k]));
+      if (j == i) {
+        if (!(s > 0.0f)) return i + 1;
+        ri[i] = R(sqrtf(s));
+      } else {
+        ri[j] = R(s / rj[j]);
+      }
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) a[(size_t)i * n + j] = 0.0f;
+  return 0;
+}
+
+/* lu_rm with Emu operations (kernels.hpp:91-119): first largest |a_ik| by
+ * strict >, whole-row swaps, a zero pivot records info and skips the column. */
+static int emu_lu_rm(float *a, int n, int *piv, orc_rnd R) {
+  int info = 0;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    float best = a[(size_t)k * n + k] < 0.0f ? -a[(size_t)k * n + k] : a[(size_t)k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      const float x = a[(size_t)i * n + k], v = x < 0.0f ? -x : x;
+      if (v > best) {
+        best = v;
+        p = i;
+      }
+    }
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; ++j) {
+        const float t = a[(size_t)k * n + j];
+        a[(size_t)k * n + j] = a[(size_t)p * n + j];
+        a[(size_t)p * n + j] = t;
+      }
+    const float pk = a[(size_t)k * n + k];
+    if (pk == 0.0f) {
+      if (info == 0) info = k + 1;
+      continue;
+    }
+    const float *rk = a + (size_t)k * n;
+    for (int i = k + 1; i < n; ++i) {
+      float *ri = a + (size_t)i * n;
+      const float f = R(ri[k] / pk);
+      ri[k] = f;
+      for (int j = k + 1; j < n; ++j) ri[j] = R(ri[j] - R(f * rk[j]));
+    }
+  }
+  return info;
+}
+
+/* cholesky_solve_rm with Emu operations (kernels.hpp:184-202): forward rows
+ * i ascending (updates k ascending, then the division), backward rows i
+ * descending with the updates k = i+1 .. n-1 ascending. */
+static void emu_chol_solve_rm(const float *L, int n, float *B, int nrhs, orc_rnd R) {
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < i; ++k)
+      for (int r = 0; r < nrhs; ++r)
+        B[(size_t)i * nrhs + r] = R(B[(size_t)i * nrhs + r] - R(L[(size_t)i * n + k] * B[(size_t)k * nrhs + r]));
+    for (int r = 0; r < nrhs; ++r) B[(size_t)i * nrhs + r] = R(B[(size_t)i * nrhs + r] / L[(size_t)i * n + i]);
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    for (int k = i + 1; k < n; ++k)
+      for (int r = 0; r < nrhs; ++r)
+        B[(size_t)i * nrhs + r] = R(B[(size_t)i * nrhs + r] - R(L[(size_t)k * n + i] * B[(size_t)k * nrhs + r]));
+    for (int r = 0; r < nrhs; ++r) B[(size_t)i * nrhs + r] = R(B[(size_t)i * nrhs + r] / L[(size_t)i * n + i]);
+  }
+}
+
+/* lu_solve_rm with Emu operations (kernels.hpp:204-227): the swaps piv[0..n-1]
+ * in order, unit-lower forward pass, upper backward pass as above. */
+static void emu_lu_solve_rm(const float *LU, const int *piv, int n, float *B, int nrhs, orc_rnd R) {
+  for (int k = 0; k < n; ++k)
+    if (piv[k] != k)
+      for (int r = 0; r < nrhs; ++r) {
+        const float t = B[(size_t)k * nrhs + r];
+        B[(size_t)k * nrhs + r] = B[(size_t)piv[k] * nrhs + r];
+        B[(size_t)piv[k] * nrhs + r] = t;
+      }
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < i; ++k)
+      for (int r = 0; r < nrhs; ++r)
+        B[(size_t)i * nrhs + r] = R(B[(size_t)i * nrhs + r] - R(LU[(size_t)i * n + k] * B[(size_t)k * nrhs + r]));
+  for (int i = n - 1; i >= 0; --i) {
+    for (int k = i + 1; k < n; ++k)
+      for (int r = 0; r < nrhs; ++r)
+        B[(size_t)i * nrhs + r] = R(B[(size_t)i * nrhs + r] - R(LU[(size_t)i * n + k] * B[(size_t)k * nrhs + r]));
+    for (int r = 0; r < nrhs; ++r) B[(size_t)i * nrhs + r] = R(B[(size_t)i * nrhs + r] / LU[(size_t)i * n + i]);
+  }
+}
+
+static orc_rnd emu_rounding(int p) { return p == 2 ? orc_round_fp16 : p == 3 ? orc_round_bf16 : NULL; }
+
+/* Batched runtime-precision entry points for float (f32) and double (f64)
+ * storage: copy in with rounding, run, copy the float values back. */
+#define ORC_EMU_TYPED(S, SUF)                                                                              \
+  int orc_emu_cholesky_##SUF(int p, S *A, int n, int64_t batch, int *info) {                                \
+    const orc_rnd R = emu_rounding(p);                                                                     \
+    if (!R || orc_check_dims(n, batch)) return ORC_EINVAL;                                                 \
+    const int64_t nn = (int64_t)n * n;                                                                     \
+    _Pragma("omp parallel for schedule(static) num_threads(orc_threads_for(batch))")                     \
+    for (int64_t b = 0; b < batch; ++b) {                                                                  \
+      float a[ORC_MAX_N * ORC_MAX_N];                                                                      \
+      for (int64_t k = 0; k < nn; ++k) a[k] = R((float)A[b * nn + k]);                                    \
+      const int r = emu_chol_rm(a, n, R);                                                                  \
+      if (info) info[b] = r;                                                                               \
+      for (int64_t k = 0; k < nn; ++k) A[b * nn + k] = (S)a[k];                                           \
+    }                                                                                                      \
+    return 0;                                                                                              \
+  }                                                                                                        \
+  int orc_emu_lu_##SUF(int p, S *A, int n, int64_t batch, int *piv, int *info) {                            \
+    const orc_rnd R = emu_rounding(p);                                                                     \
+    if (!R || orc_check_dims(n, batch) || !piv) return ORC_EINVAL;                                         \
+    const int64_t nn = (int64_t)n * n;                                                                     \
+    _Pragma("omp parallel for schedule(static) num_threads(orc_threads_for(batch))")                     \
+    for (int64_t b = 0; b < batch; ++b) {                                                                  \
+      float a[ORC_MAX_N * ORC_MAX_N];                                                                      \
+      for (int64_t k = 0; k < nn; ++k) a[k] = R((float)A[b * nn + k]);                                    \
+      const int r = emu_lu_rm(a, n, piv + b * n, R);                                                       \
+      if (info) info[b] = r;                                                                               \
+      for (int64_t k = 0; k < nn; ++k) A[b * nn + k] = (S)a[k];                                           \
+    }                                                                                                      \
+    return 0;                                                                                              \
+  }                                                                                                        \
+  int orc_emu_chol_solve_##SUF(int p, const S *L, int n, int64_t batch, S *B, int nrhs) {                   \
+    const orc_rnd R = emu_rounding(p);                                                                     \
+    if (!R || orc_check_dims(n, batch) || nrhs < 0 || nrhs > ORC_EMU_MAX_NRHS) return ORC_EINVAL;         \
+    const int64_t nn = (int64_t)n * n, bn = (int64_t)n * nrhs;                                             \
+    _Pragma("omp parallel for schedule(static) num_threads(orc_threads_for(batch))")                     \
+    for (int64_t b = 0; b < batch; ++b) {                                                                  \
+      float l[ORC_MAX_N * ORC_MAX_N], x[ORC_MAX_N * ORC_EMU_MAX_NRHS];                                     \
+      for (int64_t k = 0; k < nn; ++k) l[k] = R((float)L[b * nn + k]);                                    \
+      for (int64_t k = 0; k < bn; ++k) x[k] = R((float)B[b * bn + k]);                                    \
+      emu_chol_solve_rm(l, n, x, nrhs, R);                                                                 \
+      for (int64_t k = 0; k < bn; ++k) B[b * bn + k] = (S)x[k];                                           \
+    }                                                                                                      \
+    return 0;                                                                                              \
+  }                                                                                                        \
+  int orc_emu_lu_solve_##SUF(int p, const S *LU, const int *piv, int n, int64_t batch, S *B, int nrhs) {    \
+    const orc_rnd R = emu_rounding(p);                                                                     \
+    if (!R || orc_check_dims(n, batch) || !piv || nrhs < 0 || nrhs > ORC_EMU_MAX_NRHS) return ORC_EINVAL; \
+    const int64_t nn = (int64_t)n * n, bn = (int64_t)n * nrhs;                                             \
+    _Pragma("omp parallel for schedule(static) num_threads(orc_threads_for(batch))")                     \
+    for (int64_t b = 0; b < batch; ++b) {                                                                  \
+      float l[ORC_MAX_N * ORC_MAX_N], x[ORC_MAX_N * ORC_EMU_MAX_NRHS];                                     \
+      for (int64_t k = 0; k < nn; ++k) l[k] = R((float)LU[b * nn + k]);                                   \
+      for (int64_t k = 0; k < bn; ++k) x[k] = R((float)B[b * bn + k]);                                    \
+      emu_lu_solve_rm(l, piv + b * n, n, x, nrhs, R);                                                      \
+      for (int64_t k = 0; k < bn; ++k) B[b * bn + k] = (S)x[k];                                           \
+    }                                                                                                      \
+    return 0;                                                                                              \
+  }
+
+#define ORC_EMU_MAX_NRHS 16 /* oracle limit (stack buffers), not the reference's */
+ORC_EMU_TYPED(float, f32)
+ORC_EMU_TYPED(double, f64)
+#undef ORC_EMU_TYPED
 
 /* ------------------------------------------------ long-double ground truth */
 typedef long double LD;

@@ -3,7 +3,7 @@
 Run in a container that holds the reference sources (they are compiled, where
 they lie, by `make -C oracle ref` into oracle/_ref/libtsdx_ref.so):
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [--only emulated]
 
 Every fixture stores the raw input bytes together with the reference's outputs
 (factors, pivots, info, solutions, metric values), so the tests never need to
@@ -91,7 +91,41 @@ def factor_cases(rng, dt):
     return chol, lu
 
 
+def emulated(R):
+    """Runtime-precision calls in fp16 / bf16 (the reference's Emu types under
+    its Auto backend, i.e. the scalar kernels) on float32 and float64 storage:
+    factors with their failure content, pivots, info and both solves."""
+    rng = np.random.default_rng(20261204)
+    out = {}
+    for prec in ("fp16", "bf16"):
+        for dt, sfx, sizes in ((np.float32, "f32", (1, 2, 3, 5, 8, 13, 16, 24, 33, 64)),
+                               (np.float64, "f64", (3, 16, 40))):
+            for n in sizes:
+                b = 5
+                S = spd(rng, b, n, 1e2).astype(dt)
+                if n >= 3:
+                    S[1, 2, 2] = -1.0                  # fails at pivot 3 or earlier
+                S[2] *= 3e4                            # fp16: entries near the top of the range
+                G = rng.standard_normal((b, n, n)).astype(dt)
+                if n >= 2:
+                    G[3, :, n // 2] = 0                # zero pivot: info n//2 + 1, column skipped
+                G[4, 0, 0] = np.nan
+                B = rng.standard_normal((b, n, 3)).astype(dt)
+                key = f"{prec}_{sfx}_n{n}"
+                L, info = R.cholesky_p(S, prec)
+                LU, piv, linfo = R.lu_p(G, prec)
+                out.update({f"{key}_S": S, f"{key}_L": L, f"{key}_info": info,
+                            f"{key}_G": G, f"{key}_LU": LU, f"{key}_piv": piv, f"{key}_linfo": linfo,
+                            f"{key}_B": B,
+                            f"{key}_X": R.cholesky_solve_p(L, B, prec),
+                            f"{key}_Y": R.lu_solve_p(LU, piv, B, prec)})
+    np.savez_compressed(OUT / "emulated.npz", **out)
+
+
 def main():
+    if sys.argv[1:] == ["--only", "emulated"]:
+        emulated(Oracle("reference"))
+        return
     R = Oracle("reference")
     rng = np.random.default_rng(20261203)
     for dt, sfx in ((np.float32, "f32"), (np.float64, "f64")):
@@ -153,6 +187,7 @@ def main():
     for p in ("fp32", "fp16", "bf16", "fp64"):
         out[f"round_to_{p}"] = R.round_to(x64, p)
     np.savez_compressed(OUT / "precision.npz", **out)
+    emulated(R)
     for f in sorted(OUT.glob("*.npz")):
         print(f.name, f.stat().st_size)
 

@@ -66,3 +66,56 @@ def gesv(A, B):
     _lib.call(f"tsdx_hip_gesv_batched_{_t(A.dtype)}", _p(A), A.shape[-1], A.shape[0], _p(B), B.shape[-1], _p(piv),
               _p(info))
     return A, B, piv, info
+
+
+# ------------------------------------------- emulated fp16 / bf16 (host calls)
+def cholesky_emu(A, prec):
+    A = np.array(A, copy=True, order="C")
+    info = np.zeros(A.shape[0], np.int32)
+    _lib.call(f"tsdx_hip_cholesky_emu_batched_{_t(A.dtype)}", _p(A), A.shape[-1], A.shape[0], _p(info), _lib.EMU[prec])
+    return A, info
+
+
+def lu_emu(A, prec):
+    A = np.array(A, copy=True, order="C")
+    piv = np.zeros(A.shape[:2], np.int32)
+    info = np.zeros(A.shape[0], np.int32)
+    _lib.call(f"tsdx_hip_lu_emu_batched_{_t(A.dtype)}", _p(A), A.shape[-1], A.shape[0], _p(piv), _p(info),
+              _lib.EMU[prec])
+    return A, piv, info
+
+
+def cholesky_solve_emu(L, B, prec):
+    L = np.ascontiguousarray(L)
+    B = np.array(B, copy=True, order="C", dtype=L.dtype)
+    _lib.call(f"tsdx_hip_cholesky_solve_emu_batched_{_t(L.dtype)}", _p(L), L.shape[-1], L.shape[0], _p(B),
+              B.shape[-1], _lib.EMU[prec])
+    return B
+
+
+def lu_solve_emu(LU, piv, B, prec):
+    LU = np.ascontiguousarray(LU)
+    piv = np.ascontiguousarray(piv, dtype=np.int32)
+    B = np.array(B, copy=True, order="C", dtype=LU.dtype)
+    _lib.call(f"tsdx_hip_lu_solve_emu_batched_{_t(LU.dtype)}", _p(LU), _p(piv), LU.shape[-1], LU.shape[0], _p(B),
+              B.shape[-1], _lib.EMU[prec])
+    return B
+
+
+def posv_emu(A, B, prec):
+    A = np.array(A, copy=True, order="C")
+    B = np.array(B, copy=True, order="C", dtype=A.dtype)
+    info = np.zeros(A.shape[0], np.int32)
+    _lib.call(f"tsdx_hip_posv_emu_batched_{_t(A.dtype)}", _p(A), A.shape[-1], A.shape[0], _p(B), B.shape[-1],
+              _p(info), _lib.EMU[prec])
+    return A, B, info
+
+
+def gesv_emu(A, B, prec):
+    A = np.array(A, copy=True, order="C")
+    B = np.array(B, copy=True, order="C", dtype=A.dtype)
+    piv = np.zeros(A.shape[:2], np.int32)
+    info = np.zeros(A.shape[0], np.int32)
+    _lib.call(f"tsdx_hip_gesv_emu_batched_{_t(A.dtype)}", _p(A), A.shape[-1], A.shape[0], _p(B), B.shape[-1],
+              _p(piv), _p(info), _lib.EMU[prec])
+    return A, B, piv, info

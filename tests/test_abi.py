@@ -100,8 +100,9 @@ def test_python_error_types(tsdx):
             tsdx.lu(np.eye(3)[None], backend=be)
     with pytest.raises(ValueError):
         tsdx.lu(np.eye(3)[None], backend="tpu")
-    with pytest.raises(RuntimeError, match="emulated"):
-        tsdx.cholesky(np.eye(3, dtype=np.float32)[None], precision="bf16")
+    for be in ("naive", "scalar", "simd"):  # emulated formats take the same backend rules
+        with pytest.raises(RuntimeError, match="CPU backend"):
+            tsdx.cholesky(np.eye(3, dtype=np.float32)[None], precision="bf16", backend=be)
 
 
 def test_no_device_fails_loudly(tsdx):
@@ -111,6 +112,8 @@ def test_no_device_fails_loudly(tsdx):
         tsdx.cholesky(np.eye(3)[None])
     with pytest.raises(RuntimeError, match="no HIP device"):
         tsdx.gesv(np.eye(3)[None], np.ones((1, 3)))
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        tsdx.cholesky(np.eye(3, dtype=np.float16)[None])           # emulated fp16: no CPU path either
 
 
 def test_round_to_and_unit_roundoff_match_reference(tsdx):

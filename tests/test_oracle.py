@@ -101,6 +101,78 @@ def test_oracle_equals_compiled_reference(oracle, ref_lib, dt):
         assert bits_equal(oracle.gesv(G, B)[1], ref_lib.gesv(G, B)[1]), n
 
 
+# -------------------------------------------------- emulated fp16 / bf16 (Emu<F>)
+def _emu_keys():
+    g = golden("emulated")
+    return g, golden_cases(g, "_S")
+
+
+def test_emulated_matches_golden(oracle):
+    """Runtime-precision fp16 / bf16 calls of the port against the compiled
+    reference's outputs: factors (failure content included), pivots, info and
+    both solves, bit for bit."""
+    g, keys = _emu_keys()
+    assert len(keys) == 26
+    for key in keys:
+        prec = key.split("_")[0]
+        L, info = oracle.cholesky_p(g[f"{key}_S"], prec)
+        assert bits_equal(L, g[f"{key}_L"]) and np.array_equal(info, g[f"{key}_info"]), \
+            (key, first_bit_mismatch(L, g[f"{key}_L"]))
+        LU, piv, linfo = oracle.lu_p(g[f"{key}_G"], prec)
+        assert bits_equal(LU, g[f"{key}_LU"]) and np.array_equal(piv, g[f"{key}_piv"]), key
+        assert np.array_equal(linfo, g[f"{key}_linfo"]), key
+        assert bits_equal(oracle.cholesky_solve_p(g[f"{key}_L"], g[f"{key}_B"], prec), g[f"{key}_X"]), key
+        assert bits_equal(oracle.lu_solve_p(g[f"{key}_LU"], g[f"{key}_piv"], g[f"{key}_B"], prec), g[f"{key}_Y"]), key
+
+
+def test_emulated_fixture_cases_cover_failures():
+    g, keys = _emu_keys()
+    assert any((g[f"{k}_info"] > 0).any() for k in keys)          # failed Cholesky
+    assert any((g[f"{k}_linfo"] > 0).any() for k in keys)         # zero LU pivot
+    assert any(np.isnan(g[f"{k}_LU"]).any() for k in keys)        # NaN propagation
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_emulated_equals_compiled_reference(oracle, ref_lib, dt):
+    rng = np.random.default_rng(12)
+    for prec in ("fp16", "bf16"):
+        for n in (1, 2, 4, 7, 16, 21, 32, 50, 64):
+            G = rng.standard_normal((7, n, n)).astype(dt)
+            M = rng.standard_normal((7, n, n))
+            S = (M @ M.transpose(0, 2, 1) / n + np.eye(n)).astype(dt)
+            S[3] = -S[3]
+            L1, i1 = oracle.cholesky_p(S, prec)
+            L2, i2 = ref_lib.cholesky_p(S, prec)
+            assert bits_equal(L1, L2) and np.array_equal(i1, i2), (prec, n)
+            F1, p1, j1 = oracle.lu_p(G, prec)
+            F2, p2, j2 = ref_lib.lu_p(G, prec)
+            assert bits_equal(F1, F2) and np.array_equal(p1, p2) and np.array_equal(j1, j2), (prec, n)
+            B = rng.standard_normal((7, n, 2)).astype(dt)
+            assert bits_equal(oracle.cholesky_solve_p(L2, B, prec), ref_lib.cholesky_solve_p(L2, B, prec))
+            assert bits_equal(oracle.lu_solve_p(F2, p2, B, prec), ref_lib.lu_solve_p(F2, p2, B, prec))
+
+
+def test_emulated_closed_forms(oracle):
+    """Values that every step keeps exact in the format give the exact factors;
+    a failure leaves the rows after it as the rounded input; the upper
+    triangle is zeroed only on success."""
+    A = np.array([[[4.0, 2.0, -2.0], [2.0, 10.0, 5.0], [-2.0, 5.0, 11.0]]], np.float32)
+    for prec in ("fp16", "bf16"):
+        L, info = oracle.cholesky_p(A, prec)
+        assert info[0] == 0
+        assert L[0, 0, 0] == 2 and L[0, 1, 0] == 1 and L[0, 1, 1] == 3 and L[0, 2, 0] == -1 and L[0, 2, 1] == 2
+        assert L[0, 2, 2] == oracle.round_to(np.sqrt(6.0), prec)[()] and (np.triu(L[0], 1) == 0).all()
+        M = np.array([[[0.0, 2.0], [4.0, 8.0]]], np.float32)
+        LU, piv, info = oracle.lu_p(M, prec)
+        assert list(piv[0]) == [1, 1] and info[0] == 0
+        np.testing.assert_array_equal(LU[0], [[4.0, 8.0], [0.0, 2.0]])
+        F = np.array([[[1.0, 0.1], [0.1, -1.0]]], np.float32)   # second pivot fails
+        L, info = oracle.cholesky_p(F, prec)
+        r = oracle.round_to(F[0].astype(np.float64), prec)
+        assert info[0] == 2 and L[0, 0, 0] == 1 and L[0, 1, 0] == r[1, 0]
+        assert L[0, 0, 1] == r[0, 1] and L[0, 1, 1] == r[1, 1]    # untouched entries come back rounded
+
+
 # ------------------------------------------------------------------ closed forms
 def test_closed_forms(oracle):
     L, info = oracle.cholesky(np.array([[[4.0, 2.0], [2.0, 3.0]]]))

@@ -13,6 +13,7 @@
 
 #include "../../include/tsdx_hip.h"
 #include "capi_internal.hpp"
+#include "capi_staging.hpp"
 #include "launch.hpp"
 #include "mixed_api.hpp"
 
@@ -46,15 +47,6 @@ void require_device() {
 template <class T>
 int vec_ok(const void* p, int n) {
   return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && ((size_t)n * sizeof(T)) % 16 == 0;
-}
-
-template <class F>
-hipError_t with_bucket(int n, F&& f) {
-  if (n <= 4) return f(ic_t<4>{});
-  if (n <= 8) return f(ic_t<8>{});
-  if (n <= 16) return f(ic_t<16>{});
-  if (n <= 32) return f(ic_t<32>{});
-  return f(ic_t<64>{});
 }
 
 // ------------------------------------------------------------ device calls
@@ -180,53 +172,10 @@ void posv_ir(const double* A, int n, int64_t batch, const double* B, double* X, 
   hip_check(launch_posv_ir(a, low, s), "posv_ir launch");
 }
 
-// -------------------------------------------------------- host staging
-// Grow-only device buffers and one library stream per device, used by the
-// synchronous host-pointer calls.  Calls are serialised per process.
-struct Staging {
-  std::mutex mu;
-  struct PerDevice {
-    hipStream_t stream = nullptr;
-    void* buf[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t cap[4] = {0, 0, 0, 0};
-  };
-  std::vector<PerDevice> dev;
-};
 Staging& staging() {
   static Staging s;
   return s;
 }
-
-struct HostCall {
-  std::lock_guard<std::mutex> lock;
-  Staging::PerDevice* pd;
-  explicit HostCall(Staging& s) : lock(s.mu) {
-    require_device();
-    int d = 0;
-    hip_check(hipGetDevice(&d), "hipGetDevice");
-    if ((int)s.dev.size() <= d) s.dev.resize(d + 1);
-    pd = &s.dev[d];
-    if (!pd->stream) hip_check(hipStreamCreateWithFlags(&pd->stream, hipStreamNonBlocking), "hipStreamCreate");
-  }
-  void* buf(int slot, size_t bytes) {
-    if (bytes == 0) bytes = 16;
-    if (pd->cap[slot] < bytes) {
-      if (pd->buf[slot]) hip_check(hipFree(pd->buf[slot]), "hipFree");
-      pd->buf[slot] = nullptr;
-      pd->cap[slot] = 0;
-      hip_check(hipMalloc(&pd->buf[slot], bytes), "hipMalloc");
-      pd->cap[slot] = bytes;
-    }
-    return pd->buf[slot];
-  }
-  void h2d(void* d, const void* h, size_t bytes) {
-    if (bytes) hip_check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, pd->stream), "H2D");
-  }
-  void d2h(void* h, const void* d, size_t bytes) {
-    if (bytes) hip_check(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, pd->stream), "D2H");
-  }
-  void sync() { hip_check(hipStreamSynchronize(pd->stream), "hipStreamSynchronize"); }
-};
 
 template <class T>
 void cholesky_host(T* A, int n, int64_t batch, int* info) {

@@ -38,6 +38,16 @@ size_t ir_workspace_size(int64_t batch);
 void posv_ir(const double* A, int n, int64_t batch, const double* B, double* X, int nrhs, int factor, int max_iter,
              double tol, int* info, int* stat, void* work, size_t work_bytes, hipStream_t s);
 
+// Calls f with the size bucket of n (4, 8, 16, 32 or 64) as an ic_t.
+template <class F>
+hipError_t with_bucket(int n, F&& f) {
+  if (n <= 4) return f(ic_t<4>{});
+  if (n <= 8) return f(ic_t<8>{});
+  if (n <= 16) return f(ic_t<16>{});
+  if (n <= 32) return f(ic_t<32>{});
+  return f(ic_t<64>{});
+}
+
 // Runs f and maps exceptions to the ABI's return codes.
 template <class F>
 int guarded(F&& f) {

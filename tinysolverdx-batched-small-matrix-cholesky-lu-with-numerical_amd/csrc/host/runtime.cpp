@@ -138,17 +138,19 @@ double unit_roundoff(Precision p) {
 }
 
 // ------------------------------------------------ runtime precision
+// FP32 / FP64 run the native kernels (a converting copy when the storage type
+// differs, runtime.cpp:81-140 of the reference); FP16 / BF16 run the emulated
+// kernels of the library on the caller's storage directly (each entry rounded
+// on load, results written back as the rounded float values), which is the
+// reference's Emu copy-in / copy-out.
 namespace {
-[[noreturn]] void no_emulation(Precision p) {
-  throw std::runtime_error(std::string("emulated ") + (p == Precision::FP16 ? "fp16" : "bf16") +
-                           " precision has no HIP implementation (HIP backend unavailable for this precision)");
-}
+bool emulated(Precision p) { return p == Precision::FP16 || p == Precision::BF16; }
+int emu_format(Precision p) { return p == Precision::FP16 ? TSDX_HIP_EMU_FP16 : TSDX_HIP_EMU_BF16; }
 
 template <class F>
 void with_native(Precision p, F&& f) {
   if (p == Precision::FP32) return f(float{});
-  if (p == Precision::FP64) return f(double{});
-  no_emulation(p);
+  return f(double{});
 }
 
 template <class T, class S>
@@ -164,6 +166,10 @@ void copy_back(const std::vector<T>& v, S* x) {
 template <class S>
 void cholesky(Precision p, S* A, int n, std::int64_t batch, int* info, Backend be) {
   detail::check_dims(n, batch);
+  if (emulated(p)) {
+    detail::require_hip(be);
+    return detail::check(detail::hip_chol_emu(A, n, batch, info, emu_format(p)));
+  }
   with_native(p, [&](auto tag) {
     using T = decltype(tag);
     if constexpr (std::is_same<T, S>::value) {
@@ -179,6 +185,10 @@ void cholesky(Precision p, S* A, int n, std::int64_t batch, int* info, Backend b
 template <class S>
 void lu(Precision p, S* A, int n, std::int64_t batch, int* piv, int* info, Backend be) {
   detail::check_dims(n, batch);
+  if (emulated(p)) {
+    detail::require_hip(be);
+    return detail::check(detail::hip_lu_emu(A, n, batch, piv, info, emu_format(p)));
+  }
   with_native(p, [&](auto tag) {
     using T = decltype(tag);
     if constexpr (std::is_same<T, S>::value) {
@@ -194,6 +204,7 @@ void lu(Precision p, S* A, int n, std::int64_t batch, int* piv, int* info, Backe
 template <class S>
 void cholesky_solve(Precision p, const S* L, int n, std::int64_t batch, S* B, int nrhs) {
   detail::check_dims(n, batch);
+  if (emulated(p)) return detail::check(detail::hip_chol_solve_emu(L, n, batch, B, nrhs, emu_format(p)));
   with_native(p, [&](auto tag) {
     using T = decltype(tag);
     if constexpr (std::is_same<T, S>::value) {
@@ -210,6 +221,7 @@ void cholesky_solve(Precision p, const S* L, int n, std::int64_t batch, S* B, in
 template <class S>
 void lu_solve(Precision p, const S* LU, const int* piv, int n, std::int64_t batch, S* B, int nrhs) {
   detail::check_dims(n, batch);
+  if (emulated(p)) return detail::check(detail::hip_lu_solve_emu(LU, piv, n, batch, B, nrhs, emu_format(p)));
   with_native(p, [&](auto tag) {
     using T = decltype(tag);
     if constexpr (std::is_same<T, S>::value) {
@@ -226,6 +238,10 @@ void lu_solve(Precision p, const S* LU, const int* piv, int n, std::int64_t batc
 template <class S>
 void posv(Precision p, S* A, int n, std::int64_t batch, S* B, int nrhs, int* info, Backend be) {
   detail::check_dims(n, batch);
+  if (emulated(p)) {
+    detail::require_hip(be);
+    return detail::check(detail::hip_posv_emu(A, n, batch, B, nrhs, info, emu_format(p)));
+  }
   with_native(p, [&](auto tag) {
     using T = decltype(tag);
     if constexpr (std::is_same<T, S>::value) {
@@ -243,6 +259,10 @@ void posv(Precision p, S* A, int n, std::int64_t batch, S* B, int nrhs, int* inf
 template <class S>
 void gesv(Precision p, S* A, int n, std::int64_t batch, S* B, int nrhs, int* piv, int* info, Backend be) {
   detail::check_dims(n, batch);
+  if (emulated(p)) {
+    detail::require_hip(be);
+    return detail::check(detail::hip_gesv_emu(A, n, batch, B, nrhs, piv, info, emu_format(p)));
+  }
   with_native(p, [&](auto tag) {
     using T = decltype(tag);
     if constexpr (std::is_same<T, S>::value) {

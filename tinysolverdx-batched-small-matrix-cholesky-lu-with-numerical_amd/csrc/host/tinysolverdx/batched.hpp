@@ -91,6 +91,44 @@ inline int hip_gesv(double* A, int n, std::int64_t b, double* B, int r, int* p, 
   return tsdx_hip_gesv_batched_f64(A, n, b, B, r, p, i);
 }
 
+// Emulated fp16 / bf16 (fmt: TSDX_HIP_EMU_FP16 / _BF16) on float32 or float64 storage.
+inline int hip_chol_emu(float* A, int n, std::int64_t b, int* i, int f) {
+  return tsdx_hip_cholesky_emu_batched_f32(A, n, b, i, f);
+}
+inline int hip_chol_emu(double* A, int n, std::int64_t b, int* i, int f) {
+  return tsdx_hip_cholesky_emu_batched_f64(A, n, b, i, f);
+}
+inline int hip_lu_emu(float* A, int n, std::int64_t b, int* p, int* i, int f) {
+  return tsdx_hip_lu_emu_batched_f32(A, n, b, p, i, f);
+}
+inline int hip_lu_emu(double* A, int n, std::int64_t b, int* p, int* i, int f) {
+  return tsdx_hip_lu_emu_batched_f64(A, n, b, p, i, f);
+}
+inline int hip_chol_solve_emu(const float* L, int n, std::int64_t b, float* B, int r, int f) {
+  return tsdx_hip_cholesky_solve_emu_batched_f32(L, n, b, B, r, f);
+}
+inline int hip_chol_solve_emu(const double* L, int n, std::int64_t b, double* B, int r, int f) {
+  return tsdx_hip_cholesky_solve_emu_batched_f64(L, n, b, B, r, f);
+}
+inline int hip_lu_solve_emu(const float* LU, const int* p, int n, std::int64_t b, float* B, int r, int f) {
+  return tsdx_hip_lu_solve_emu_batched_f32(LU, p, n, b, B, r, f);
+}
+inline int hip_lu_solve_emu(const double* LU, const int* p, int n, std::int64_t b, double* B, int r, int f) {
+  return tsdx_hip_lu_solve_emu_batched_f64(LU, p, n, b, B, r, f);
+}
+inline int hip_posv_emu(float* A, int n, std::int64_t b, float* B, int r, int* i, int f) {
+  return tsdx_hip_posv_emu_batched_f32(A, n, b, B, r, i, f);
+}
+inline int hip_posv_emu(double* A, int n, std::int64_t b, double* B, int r, int* i, int f) {
+  return tsdx_hip_posv_emu_batched_f64(A, n, b, B, r, i, f);
+}
+inline int hip_gesv_emu(float* A, int n, std::int64_t b, float* B, int r, int* p, int* i, int f) {
+  return tsdx_hip_gesv_emu_batched_f32(A, n, b, B, r, p, i, f);
+}
+inline int hip_gesv_emu(double* A, int n, std::int64_t b, double* B, int r, int* p, int* i, int f) {
+  return tsdx_hip_gesv_emu_batched_f64(A, n, b, B, r, p, i, f);
+}
+
 template <class T>
 void require_native() {
   static_assert(is_native<T>::value, "the HIP backend computes in float or double");

@@ -19,9 +19,12 @@ void set_num_threads(int n);  // n <= 0 restores the default (TSDX_NUM_THREADS o
 
 // Runtime-precision API (reference runtime.cpp:95-150).  S is the storage type
 // (float or double).  FP32/FP64 run on the HIP backend, in place when the
-// storage already has that type, through a converting copy otherwise.  The
-// reference's emulated FP16/BF16 formats have no HIP implementation and raise
-// std::runtime_error (as the reference's CUDA backend does).
+// storage already has that type, through a converting copy otherwise.  FP16 /
+// BF16 run the emulated kernels of the HIP backend (the reference's Emu<F>
+// arithmetic under its Auto backend, precision.hpp:71-97): every entry is
+// rounded to the format on the way in, every operation rounds its result, and
+// the rounded values come back in S, bit-identical to the reference.  posv /
+// gesv are the factor followed by the solve, as posv_batched / gesv_batched.
 template <class S>
 void cholesky(Precision p, S* A, int n, std::int64_t batch, int* info, Backend be = Backend::Auto);
 template <class S>

@@ -37,6 +37,19 @@ for _t in ("f32", "f64"):
         f"tsdx_hip_posv_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp],
         f"tsdx_hip_gesv_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp, _vp],
     })
+for _t in ("f32", "f64"):
+    SIGNATURES.update({
+        f"tsdx_hip_potrf_emu_{_t}": [_vp, _i, _i64, _vp, _i, _vp],
+        f"tsdx_hip_getrf_emu_{_t}": [_vp, _i, _i64, _vp, _vp, _i, _vp],
+        f"tsdx_hip_potrs_emu_{_t}": [_vp, _i, _i64, _vp, _i, _i, _vp],
+        f"tsdx_hip_getrs_emu_{_t}": [_vp, _vp, _i, _i64, _vp, _i, _i, _vp],
+        f"tsdx_hip_cholesky_emu_batched_{_t}": [_vp, _i, _i64, _vp, _i],
+        f"tsdx_hip_lu_emu_batched_{_t}": [_vp, _i, _i64, _vp, _vp, _i],
+        f"tsdx_hip_cholesky_solve_emu_batched_{_t}": [_vp, _i, _i64, _vp, _i, _i],
+        f"tsdx_hip_lu_solve_emu_batched_{_t}": [_vp, _vp, _i, _i64, _vp, _i, _i],
+        f"tsdx_hip_posv_emu_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp, _i],
+        f"tsdx_hip_gesv_emu_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp, _vp, _i],
+    })
 SIGNATURES["tsdx_hip_posv_ir_workspace_size"] = [_i64]
 SIGNATURES["tsdx_hip_posv_ir_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, C.c_size_t, _vp]
 SIGNATURES["tsdx_hip_posv_ir_noalloc_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp]
@@ -66,6 +79,8 @@ _RES = {"tsdx_hip_last_error": C.c_char_p, "tsdx_hip_posv_ir_workspace_size": C.
         "tsdx_hip_posv_routed_workspace_size": C.c_size_t, "tsdx_hip_router_infer_workspace_size": C.c_size_t}
 
 KIND = {"cholesky": 0, "lu": 1}
+
+EMU = {"fp16": 1, "bf16": 2}  # TSDX_HIP_EMU_FP16 / _BF16
 
 FACTOR = {"auto": 0, "fp32": 1, "bf16": 2, "fp16": 3, "fp64": 4, "fp64_block": 5}
 

@@ -8,8 +8,9 @@ the host API (reference_solve, chol/lu/solve backward errors, solve forward
 error; runtime.cpp:162-260 in the reference).  What is measured here is what
 the GPU computes:
 
-* plain fp32 / fp64 Cholesky (posv) and LU (gesv) with backend="hip"
-  (the reference's CPU emulation of fp16 / bf16 has no GPU path);
+* plain Cholesky (posv) and LU (gesv) with backend="hip" in fp64, fp32 and
+  the emulated bf16 / fp16 (every operation rounded to the format, the
+  reference's Emu arithmetic, computed by the GPU kernels);
 * the posv tiers: fp32 / bf16 / fp16 factor + float64 refinement with the
   float64 fallback, and the routed solve (the shipped router), for which the
   "failure" is the fraction of matrices that needed the float64 solve.
@@ -28,7 +29,7 @@ import numpy as np
 import tinysolverdx as tsdx
 from tinysolverdx import matrices
 
-PLAIN = ["fp32", "fp64"]
+PLAIN = ["bf16", "fp16", "fp32", "fp64"]
 TIERS = ["fp32+IR", "bf16+IR", "fp16+IR", "routed"]
 
 
