@@ -50,6 +50,20 @@ const char *tsdx_hip_last_error(void);
 /* Library version as major*10000 + minor*100 + patch. */
 int tsdx_hip_version(void);
 
+/* Device-memory helpers (no reference counterpart: the reference's API takes
+ * host pointers only, README.md:99-101).  For callers that hold device arrays
+ * without a GPU framework of their own, e.g. the Python module's device-array
+ * path.  tsdx_hip_malloc allocates on the current device; tsdx_hip_memcpy is
+ * synchronous and takes any pair of host / device pointers;
+ * tsdx_hip_pointer_device gives the device of a device (or managed) pointer and
+ * fails with TSDX_HIP_EINVAL for host memory. */
+int tsdx_hip_malloc(void **ptr, size_t bytes);
+int tsdx_hip_free(void *ptr);
+int tsdx_hip_memcpy(void *dst, const void *src, size_t bytes);
+int tsdx_hip_device_synchronize(void);
+int tsdx_hip_get_device(int *device);
+int tsdx_hip_pointer_device(const void *ptr, int *device);
+
 /* ---------------------------------------------------------------------------
  * Device-pointer entry points (stream-ordered).
  * ------------------------------------------------------------------------ */

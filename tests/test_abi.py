@@ -115,6 +115,22 @@ def test_no_device_fails_loudly(tsdx):
     with pytest.raises(RuntimeError, match="no HIP device"):
         tsdx.cholesky(np.eye(3, dtype=np.float16)[None])           # emulated fp16: no CPU path either
 
+    class DeviceLike:  # an array-interface producer (the pointer is never dereferenced here)
+        __cuda_array_interface__ = dict(shape=(2, 3, 3), typestr="<f4", data=(4096, False), strides=None, version=2)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        tsdx.cholesky(DeviceLike())
+
+
+def test_device_array_detection(tsdx):
+    import torch
+    assert not tsdx._core.is_device_array(np.eye(3))
+    assert not tsdx._core.is_device_array(torch.eye(3))            # CPU tensors stay on the host path
+
+    class DeviceLike:
+        __cuda_array_interface__ = dict(shape=(3, 3), typestr="<f8", data=(4096, False), version=3)
+    assert tsdx._core.is_device_array(DeviceLike())
+    assert tsdx.DeviceArray.__name__ == "DeviceArray"
+
 
 def test_round_to_and_unit_roundoff_match_reference(tsdx):
     g = golden("precision")

@@ -111,7 +111,7 @@ def build_core(verbose=False):
     cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-ffp-contract=off", "-pthread",
            f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}", f"-I{host}",
            f"-I{ROOT / 'include'}",
-           str(host / "runtime.cpp"), str(host / "bindings.cpp"),
+           str(host / "runtime.cpp"), str(host / "bindings.cpp"), str(host / "device_arrays.cpp"),
            f"-L{PKG}", "-ltsdx_hip", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
     _run(cmd, verbose)
     os.replace(tmp, out)

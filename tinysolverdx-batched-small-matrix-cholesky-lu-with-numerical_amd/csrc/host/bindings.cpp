@@ -6,7 +6,8 @@
 // the backend names, has_hip(), and the fused posv / gesv drivers, whose
 // counterparts the reference composes from two calls.  Each call works on
 // fresh output arrays (inputs are never written) and releases the GIL while
-// the device works.  The machine probes measure the GPU.
+// the device works.  The machine probes measure the GPU.  The *_device
+// functions (device_arrays.cpp) take arrays already in GPU memory.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 
@@ -250,11 +251,16 @@ void add_metrics(py::module_& m) {
 
 }  // namespace
 
+namespace tsdx_py {
+void add_device_arrays(py::module_& m);  // device_arrays.cpp
+}
+
 PYBIND11_MODULE(_core, m) {
   m.doc() = "tinysolverdx-hip native core: batched small-matrix Cholesky/LU on MI355X (gfx950)";
   add_solvers<float>(m, "f32");
   add_solvers<double>(m, "f64");
   add_metrics(m);
+  tsdx_py::add_device_arrays(m);
 
   m.def("has_hip", &tsdx::has_hip);
   m.def("has_cuda", &tsdx::has_cuda);

@@ -10,9 +10,18 @@ bfloat16 values).  Inputs are never modified.  The work runs on the GPU
 "simd") are not part of this build and raise RuntimeError.  round_to and the
 long-double error metrics run on the host, as in the reference.
 
+Device arrays: every solver also takes matrices that already live in GPU
+memory, from any producer of __cuda_array_interface__ or DLPack (torch
+tensors, CuPy / Numba arrays, DeviceArray), without a trip over PCIe.  Then
+the results are DeviceArray objects (library-owned GPU memory that exports
+both protocols: torch.from_dlpack(x) or cupy.asarray(x) take them without a
+copy, x.numpy() copies to the host); every array argument of the call must
+then be a device array, float32 or float64 (int32 pivots), and the precision
+is the array's own or the emulated fp16 / bf16.
+
 Beyond the reference: posv / gesv run factor and solve as one device launch,
 the machine probes measure the GPU, and `tinysolverdx.device` works on torch
-device tensors without copies.
+device tensors with stream-ordered, allocation-free calls.
 """
 import numpy as np
 
@@ -25,14 +34,14 @@ from ._core import (  # noqa: E402,F401  (re-exported)
     has_hip, has_cuda, has_openmp, num_threads, set_num_threads, unit_roundoff,
     chol_backward_error, lu_backward_error, solve_backward_error,
     solve_forward_error, reference_solve, measure_peak_gflops, measure_bandwidth_gbs,
-    measure_mfma_tflops,
+    measure_mfma_tflops, DeviceArray,
 )
 
 __all__ = ["cholesky", "lu", "cholesky_solve", "lu_solve", "posv", "gesv", "round_to",
            "has_hip", "has_cuda", "has_openmp", "num_threads", "set_num_threads", "unit_roundoff",
            "chol_backward_error", "lu_backward_error", "solve_backward_error",
            "solve_forward_error", "reference_solve", "measure_peak_gflops", "measure_bandwidth_gbs",
-           "measure_mfma_tflops"]
+           "measure_mfma_tflops", "DeviceArray"]
 
 # dtype -> precision used when none is given
 _IMPLIED = {np.dtype(np.float16): "fp16", np.dtype(np.float32): "fp32", np.dtype(np.float64): "fp64"}
@@ -92,6 +101,8 @@ class _Stack:
 
 def cholesky(A, precision=None, backend="auto"):
     """Lower Cholesky factors and LAPACK-style info (0 = ok, k>0: pivot k not positive)."""
+    if _core.is_device_array(A):
+        return _core.cholesky_device(A, precision or "", backend)
     s = _Stack(A, precision)
     L, info = s.call("cholesky", s.data, s.precision, backend)
     return s.result(L), s.flag(info)
@@ -99,6 +110,8 @@ def cholesky(A, precision=None, backend="auto"):
 
 def lu(A, precision=None, backend="auto"):
     """Packed LU with partial pivoting: (LU, piv, info). piv is scipy/LAPACK style (0-based)."""
+    if _core.is_device_array(A):
+        return _core.lu_device(A, precision or "", backend)
     s = _Stack(A, precision)
     LU, piv, info = s.call("lu", s.data, s.precision, backend)
     return s.result(LU), s.flag(piv), s.flag(info)
@@ -106,6 +119,8 @@ def lu(A, precision=None, backend="auto"):
 
 def cholesky_solve(L, B, precision=None):
     """Solve L L^T x = B given factors from :func:`cholesky`."""
+    if _core.is_device_array(L):
+        return _core.cholesky_solve_device(L, B, precision or "")
     s = _Stack(L, precision)
     b, vectors = s.rhs(B)
     return s.result(s.call("cholesky_solve", s.data, b, s.precision), vectors)
@@ -113,6 +128,8 @@ def cholesky_solve(L, B, precision=None):
 
 def lu_solve(LU, piv, B, precision=None):
     """Solve A x = B given (LU, piv) from :func:`lu`."""
+    if _core.is_device_array(LU):
+        return _core.lu_solve_device(LU, piv, B, precision or "")
     s = _Stack(LU, precision)
     p = np.asarray(piv)
     p = np.ascontiguousarray(p[np.newaxis] if s.single else p, dtype=np.int32)
@@ -122,6 +139,8 @@ def lu_solve(LU, piv, B, precision=None):
 
 def posv(A, B, precision=None, backend="auto"):
     """Solve SPD systems A x = B (factor and solve fused on the device). Returns (x, info)."""
+    if _core.is_device_array(A):
+        return _core.posv_device(A, B, precision or "", backend)
     s = _Stack(A, precision)
     b, vectors = s.rhs(B)
     _, X, info = s.call("posv", s.data, b, s.precision, backend)
@@ -130,6 +149,8 @@ def posv(A, B, precision=None, backend="auto"):
 
 def gesv(A, B, precision=None, backend="auto"):
     """Solve general systems A x = B with partial pivoting (fused on the device). Returns (x, info)."""
+    if _core.is_device_array(A):
+        return _core.gesv_device(A, B, precision or "", backend)
     s = _Stack(A, precision)
     b, vectors = s.rhs(B)
     _, _, X, info = s.call("gesv", s.data, b, s.precision, backend)

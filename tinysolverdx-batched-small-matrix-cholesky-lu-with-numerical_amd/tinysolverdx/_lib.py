@@ -50,6 +50,12 @@ for _t in ("f32", "f64"):
         f"tsdx_hip_posv_emu_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp, _i],
         f"tsdx_hip_gesv_emu_batched_{_t}": [_vp, _i, _i64, _vp, _i, _vp, _vp, _i],
     })
+SIGNATURES["tsdx_hip_malloc"] = [C.POINTER(_vp), C.c_size_t]
+SIGNATURES["tsdx_hip_free"] = [_vp]
+SIGNATURES["tsdx_hip_memcpy"] = [_vp, _vp, C.c_size_t]
+SIGNATURES["tsdx_hip_device_synchronize"] = []
+SIGNATURES["tsdx_hip_get_device"] = [C.POINTER(_i)]
+SIGNATURES["tsdx_hip_pointer_device"] = [_vp, C.POINTER(_i)]
 SIGNATURES["tsdx_hip_posv_ir_workspace_size"] = [_i64]
 SIGNATURES["tsdx_hip_posv_ir_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp, C.c_size_t, _vp]
 SIGNATURES["tsdx_hip_posv_ir_noalloc_f64"] = [_vp, _i, _i64, _vp, _vp, _i, _i, _i, _d, _vp, _vp, _vp]
