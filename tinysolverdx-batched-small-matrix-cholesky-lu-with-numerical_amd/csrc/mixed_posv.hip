@@ -49,8 +49,11 @@ template <int NB>
 static hipError_t gather(const IrArgs& a, hipStream_t s, bool plain, bool full) {
   if constexpr (NB == 64) {
     if (!plain) {
-      // two waves per SIMD fit this kernel: 4096 workgroups fill the GPU twice
-      const unsigned cap = full ? 4096u : kGatherGridShort;
+      // two waves per SIMD fit this kernel: 4096 workgroups fill the GPU twice.
+      // A short list is latency-bound (one matrix takes a wave ~25 us), so it
+      // gets one workgroup per entry up to 2048: up to about 1800 entries
+      // (7 workgroups of 20.5 KiB LDS per CU) then finish in one round.
+      const unsigned cap = full ? 4096u : 2048u;
       const dim3 gg((unsigned)(a.batch < (int64_t)cap ? a.batch : cap));
       if (a.nrhs <= 1)
         hipLaunchKernelGGL((k_posv_mx64<4, 1>), gg, dim3(kWave), 0, s, a);

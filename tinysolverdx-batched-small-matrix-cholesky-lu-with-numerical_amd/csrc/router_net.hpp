@@ -34,14 +34,24 @@ __device__ __forceinline__ v4fn ld_bias(const float* __restrict__ b, int i) {
 }
 
 // ReLU of four outputs (rows 4g..4g+3 of tile t, sample j) in bfloat16, to LDS.
+// The outputs are rounded to bfloat16 in pairs (v_cvt_pk_bf16_f32, nearest
+// even), and the ReLU is taken on the packed halves as 16-bit signed integers
+// (v_pk_max_i16 with 0): a bfloat16 with its sign bit set, -0 included, is a
+// negative integer and becomes +0, every other value is kept.  Rounding keeps
+// the sign, so this equals rounding the ReLU of the float output; a NaN output
+// with the sign bit clear stays NaN (and reaches the logits, whose NaN test
+// then picks the float64 tier).
+__device__ __forceinline__ unsigned relu_pk_bf16(float a, float b) {
+  typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  typedef short v2s __attribute__((ext_vector_type(2)));
+  const v2bf p = __builtin_convertvector((v2f){a, b}, v2bf);  // one v_cvt_pk_bf16_f32
+  const v2s r = __builtin_elementwise_max(__builtin_bit_cast(v2s, p), (v2s){0, 0});
+  return __builtin_bit_cast(unsigned, r);
+}
 __device__ __forceinline__ void net_store(v4fn h, __bf16* hs, int j, int g, int t) {
-  typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
-  v4bf q;
-  q[0] = (__bf16)(h[0] > 0.0f ? h[0] : 0.0f);
-  q[1] = (__bf16)(h[1] > 0.0f ? h[1] : 0.0f);
-  q[2] = (__bf16)(h[2] > 0.0f ? h[2] : 0.0f);
-  q[3] = (__bf16)(h[3] > 0.0f ? h[3] : 0.0f);
-  *reinterpret_cast<v4bf*>(hs + j * kNetLd + 16 * t + 4 * g) = q;
+  const uint2 q = {relu_pk_bf16(h[0], h[1]), relu_pk_bf16(h[2], h[3])};
+  *reinterpret_cast<uint2*>(hs + j * kNetLd + 16 * t + 4 * g) = q;
 }
 
 // Logits of the 16 samples: on return lane (j, 0) (l = j < 16) holds z[0..2]
