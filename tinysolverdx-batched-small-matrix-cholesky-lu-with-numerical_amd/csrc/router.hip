@@ -1,6 +1,6 @@
 // Launchers of the router kernels (router_kernels.hpp) and of the routed posv.
 #include "capi_internal.hpp"
-#include "mfma_kernels.hpp"
+#include "routed_kernels.hpp"
 #include "router_api.hpp"
 #include "router_kernels.hpp"
 #include "router_net.hpp"
