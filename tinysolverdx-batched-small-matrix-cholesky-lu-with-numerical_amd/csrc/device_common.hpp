@@ -86,17 +86,20 @@ __device__ __forceinline__ double bcast_dyn(double x, int src_lane) {
 }
 
 // ------------------------------------------------------ xor shuffles
-// M in {1, 2, 7, 15, 16, 32}: lane l receives lane l ^ M.
+// M in {1, 2, 7, 15, 16, 32}: lane l receives lane l ^ M.  The first four are
+// DPP moves inside rows of 16 (every lane reads a lane of its own row, so
+// bound_ctrl changes nothing and lets the compiler fold the move into the
+// instruction that consumes it).
 template <int M>
 __device__ __forceinline__ int xshfl_i32(int x) {
   if constexpr (M == 1) {
-    return __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
   } else if constexpr (M == 2) {
-    return __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
   } else if constexpr (M == 7) {
-    return __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    return __builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, true);  // row_half_mirror
   } else if constexpr (M == 15) {
-    return __builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false);  // row_mirror
+    return __builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, true);  // row_mirror
   } else if constexpr (M == 16) {
     return __builtin_amdgcn_ds_swizzle(x, 0x401F);  // bit mode, xor_mask 16
   } else {
@@ -117,9 +120,43 @@ __device__ __forceinline__ int xshfl(int x) {
   return xshfl_i32<M>(x);
 }
 
+// op(x of lane l, x of lane l ^ M) for M = 16, 32 on the VALU:
+// v_permlane16_swap / v_permlane32_swap of x with itself gives every lane its
+// own value and its partner's (which of the two comes first depends on the
+// lane), so for a commutative op every lane gets op(own, partner).
+template <int M>
+__device__ __forceinline__ void xpair_u32(unsigned x, unsigned& a, unsigned& b) {
+  static_assert(M == 16 || M == 32, "row pairs");
+  if constexpr (M == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    a = p[0];
+    b = p[1];
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    a = p[0];
+    b = p[1];
+  }
+}
+template <int M, class V, class Op>
+__device__ __forceinline__ V xop(V x, Op op) {
+  if constexpr (sizeof(V) == 4) {
+    unsigned a, b;
+    xpair_u32<M>(__builtin_bit_cast(unsigned, x), a, b);
+    return op(__builtin_bit_cast(V, a), __builtin_bit_cast(V, b));
+  } else {
+    static_assert(sizeof(V) == 8, "32- or 64-bit values");
+    unsigned la, lb, ha, hb;
+    xpair_u32<M>((unsigned)lo32(__builtin_bit_cast(double, x)), la, lb);
+    xpair_u32<M>((unsigned)hi32(__builtin_bit_cast(double, x)), ha, hb);
+    return op(__builtin_bit_cast(V, mk64((int)la, (int)ha)), __builtin_bit_cast(V, mk64((int)lb, (int)hb)));
+  }
+}
+
 // Butterfly all-reduce over the NB lanes of a group.  The xor masks
 // {1, 2, 7, 15, 16, 32} restricted to the group span every lane index of the
 // group, so after the last step every lane holds the reduction of the group.
+// op must be commutative (every caller's is: sums, and minima / maxima of
+// keys that hold no NaN), which the steps across rows of 16 rely on.
 template <int NB, class V, class Op>
 __device__ __forceinline__ V group_allreduce(V x, Op op) {
   static_assert(NB >= 4 && NB <= 64 && (NB & (NB - 1)) == 0, "group size");
@@ -127,8 +164,8 @@ __device__ __forceinline__ V group_allreduce(V x, Op op) {
   x = op(x, xshfl<2>(x));
   if constexpr (NB >= 8) x = op(x, xshfl<7>(x));
   if constexpr (NB >= 16) x = op(x, xshfl<15>(x));
-  if constexpr (NB >= 32) x = op(x, xshfl<16>(x));
-  if constexpr (NB >= 64) x = op(x, xshfl<32>(x));
+  if constexpr (NB >= 32) x = xop<16>(x, op);
+  if constexpr (NB >= 64) x = xop<32>(x, op);
   return x;
 }
 
@@ -146,6 +183,34 @@ template <class T>
 __device__ __forceinline__ T infT() {
   return (T)__builtin_huge_val();
 }
+
+// Pivot search of partial pivoting: whether this lane holds the largest key of
+// its group of NB lanes.  The key of a candidate row is |a| = v (a NaN counts
+// as +inf when nan_is_inf - the NaN at the current position - else as -1),
+// a row that is not a candidate -2.  For float the keys are compared as
+// 32-bit integers: the bits of a float that is not negative order like the
+// float, and -1, -2 are integers below every such key, so each step of the
+// reduction is a single DPP integer maximum.
+template <int NB, class T>
+__device__ __forceinline__ int pivot_key_i32(T v, bool cand, bool nan_is_inf) {
+  static_assert(sizeof(T) == 4, "float keys");
+  int key = (v != v) ? (nan_is_inf ? 0x7f800000 : -1) : __builtin_bit_cast(int, v);
+  return cand ? key : -2;
+}
+template <int NB, class T>
+__device__ __forceinline__ bool holds_max_key(T v, bool cand, bool nan_is_inf) {
+  if constexpr (sizeof(T) == 4) {
+    const int key = pivot_key_i32<NB>(v, cand, nan_is_inf);
+    const int mx = group_allreduce<NB>(key, [](int a, int b) { return a > b ? a : b; });
+    return cand && key == mx;
+  } else {
+    T key = (v != v) ? (nan_is_inf ? infT<T>() : T(-1)) : v;
+    key = cand ? key : T(-2);
+    const T mx = group_allreduce<NB>(key, [](T a, T b) { return a > b ? a : b; });
+    return cand && key == mx;
+  }
+}
+
 
 // Returns x unchanged but hides its value from the optimiser.  Used on the
 // lane index at the top of the right-hand-side chunk loops: otherwise LLVM
@@ -175,6 +240,22 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// A row of NB zeros in LDS that starts 32 bytes (8 of the 32 four-byte banks)
+// past `row`, modulo 128 bytes: lanes that read it in the same instruction as
+// other lanes read `row` (or, for 16-element float rows, the next group's row,
+// 64 bytes on) at the same element then never meet in a bank.  zraw holds
+// NB + 128 / sizeof(T) elements; every wave writes the zeros and waits for its
+// own writes.
+template <class T, int NB>
+__device__ __forceinline__ const T* zero_row_beside(T* zraw, const T* row) {
+  constexpr int total = NB + 128 / (int)sizeof(T);
+  for (int e = (int)(threadIdx.x % kWave); e < total; e += kWave) zraw[e] = T(0);
+  wave_sync();
+  const int off = ((int)((uintptr_t)row - (uintptr_t)zraw) + 32) & 127;
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(zraw) + off);
+}
+
 
 // ------------------------------------------------------- row load / store
 template <class T>

@@ -80,11 +80,7 @@ __device__ __forceinline__ void lu_emu(float (&r)[NB], int n, int i, int g, floa
     constexpr int k = decltype(kc)::value;
     if (k < n) {
       const bool cand = act && pos >= k;
-      const float v = __builtin_fabsf(r[k]);
-      float key = (v != v) ? (pos == k ? __builtin_huge_valf() : -1.0f) : v;
-      key = cand ? key : -2.0f;
-      const float mx = group_allreduce<NB>(key, [](float a, float b) { return a > b ? a : b; });
-      const int cp = (cand && key == mx) ? pos : 0x7fffffff;
+      const int cp = holds_max_key<NB>(__builtin_fabsf(r[k]), cand, pos == k) ? pos : 0x7fffffff;
       const int p = group_allreduce<NB>(cp, [](int a, int b) { return a < b ? a : b; });
       if (i == k) mypiv = p;
       const bool isp = act && pos == p;

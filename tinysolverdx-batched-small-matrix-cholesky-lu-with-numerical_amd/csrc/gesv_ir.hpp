@@ -34,11 +34,7 @@ __device__ __forceinline__ void lu_low(float (&r)[NB], int n, int i, int g, floa
     constexpr int k = decltype(kc)::value;
     if (k < n) {
       const bool cand = act && pos >= k;
-      const float v = __builtin_fabsf(r[k]);
-      float key = (v != v) ? -1.0f : v;
-      key = cand ? key : -2.0f;
-      const float mx = group_allreduce<NB>(key, [](float a, float b) { return a > b ? a : b; });
-      const int cp = (cand && key == mx) ? pos : 0x7fffffff;
+      const int cp = holds_max_key<NB>(__builtin_fabsf(r[k]), cand, false) ? pos : 0x7fffffff;
       const int p = group_allreduce<NB>(cp, [](int a, int b) { return a < b ? a : b; });
       const bool isp = act && pos == p;
       pos = (pos == k) ? p : (isp ? k : pos);

@@ -92,28 +92,51 @@ __device__ __forceinline__ void lu_regs(T (&r)[NB], int nloop, int n, int i, int
   mypiv = 0;
   info = 0;
   const bool act = i < n;
+  // The update without a branch (BF) measured faster for float up to n = 32
+  // and float64 up to n = 16 (1 GiB of A on MI355X: fp32 n=16 0.537 -> 0.502
+  // ms, n=32 0.807 -> 0.750 with getrf_occupancy; fp64 n=16 0.581 -> 0.514),
+  // slower for the larger float64 rows (n=32 0.684 -> 0.711, n=64 1.167 ->
+  // 1.235), which keep the branch.
+  constexpr bool BF = sizeof(T) == 4 ? NB <= 32 : NB <= 16;
+  __shared__ __align__(16) T zraw[BF ? NB + 128 / sizeof(T) : 1];
+  const T* zb = BF ? zero_row_beside<T, NB>(zraw, rowb) : nullptr;
   static_for<0, NB>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     if (k < nloop) {
       const bool cand = act && pos >= k;
-      const T v = absT(r[k]);
-      T key = (v != v) ? (pos == k ? infT<T>() : T(-1)) : v;
-      key = cand ? key : T(-2);
-      const T mx = group_allreduce<NB>(key, [](T a, T b) { return a > b ? a : b; });
-      const int cp = (cand && key == mx) ? pos : 0x7fffffff;
+      const int cp = holds_max_key<NB>(absT(r[k]), cand, pos == k) ? pos : 0x7fffffff;
       const int p = group_allreduce<NB>(cp, [](int a, int b) { return a < b ? a : b; });
       if (i == k) mypiv = p;
       const bool isp = act && pos == p;
       pos = (pos == k) ? p : (isp ? k : pos);
       T* rb = rowb + (k & 1) * kWave + g * NB;
       if (isp) {
-        static_for<k, NB>([&](auto jc) { rb[jc] = r[jc]; });
+        // from column k; for float64 from the 16-byte chunk that holds it, which
+        // lets the registers go out as the pairs they are held in (the column
+        // left of k is never read).  Either start gave the fewer register moves
+        // for its type.
+        static_for<(sizeof(T) == 8 ? k / 2 * 2 : k), NB>([&](auto jc) { rb[jc] = r[jc]; });
       }
       wave_sync();
       const T pk = rb[k];
       const bool zero = (pk == T(0));
       info = (zero && info == 0 && k < n) ? k + 1 : info;
-      if (act && pos > k && (ZTILE || !zero)) {
+      const bool upd = act && pos > k && (ZTILE || !zero);
+      if constexpr (BF) {
+        // Every lane runs the update; a row that must stay as it is reads the
+        // pivot row from zb (zeros) with the multiplier +0, and
+        // fma(-0, +0, x) = x + (-0) = x exactly for every x, signed zeros
+        // included (a divergent branch here makes the compiler copy the
+        // whole row array twice per step).
+        const T* y = upd ? rb : zb;
+        const T q = r[k] / (zero ? T(1) : pk);
+        r[k] = upd ? q : r[k];
+        const T f = upd ? q : T(0);
+        static_for<k + 1, NB>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          r[j] = fmaT(-f, y[j], r[j]);
+        });
+      } else if (upd) {
         const T f = r[k] / (zero ? T(1) : pk);
         r[k] = f;
         static_for<k + 1, NB>([&](auto jc) {
@@ -332,10 +355,20 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_potrf(T* __re
   if (t.live && t.i == 0 && info) info[t.m] = inf;
 }
 
+// Waves per SIMD the register allocation of k_getrf is asked to allow: left
+// to itself the compiler spends registers it does not need (fp32 n = 32: 107
+// VGPRs, 4 waves per SIMD; asked for 5 it fits 53, 8 waves, with no spills),
+// and the kernel is bound by the latency of its per-column chain, which more
+// waves hide.  1 = no request.
+template <class T, int NB>
+constexpr int getrf_occupancy() {
+  return NB == 32 ? 5 : 1;
+}
+
 template <class T, int NB, bool ZTILE>
-__global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_getrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ piv,
+__global__ __launch_bounds__(kWave * waves_per_block<NB>()) __attribute__((amdgpu_waves_per_eu(getrf_occupancy<T, NB>()))) void k_getrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ piv,
                                                  int* __restrict__ info, int vec) {
-  __shared__ T rowb_s[waves_per_block<NB>()][2 * kWave];
+  __shared__ __align__(16) T rowb_s[waves_per_block<NB>()][2 * kWave];
   __shared__ __align__(16) T stage_s[waves_per_block<NB>()][lu_stage_elems<T, NB>()];
   const Tile t = tile_of<NB>(batch);
   [[maybe_unused]] const int wv = t.w;

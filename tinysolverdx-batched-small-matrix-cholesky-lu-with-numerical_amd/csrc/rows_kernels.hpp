@@ -151,15 +151,22 @@ __device__ __forceinline__ void lu_rows(T (&r)[R][L * R], int nloop, int n, int 
   static_for<0, N>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     if (k < nloop) {
-      T key[R];
-      T best = T(-2);
+      // keys of the pivot rule (holds_max_key); float keys as integers
+      using K = typename std::conditional<sizeof(T) == 4, int, T>::type;
+      K key[R];
+      K best = K(-2);
       static_for<0, R>([&](auto s) {
-        const T v = absT(r[s][k]);
-        const T kk = (v != v) ? (pos[s] == k ? infT<T>() : T(-1)) : v;
-        key[s] = (act[s] && pos[s] >= k) ? kk : T(-2);
+        const bool cand = act[s] && pos[s] >= k;
+        if constexpr (sizeof(T) == 4) {
+          key[s] = pivot_key_i32<L>(absT(r[s][k]), cand, pos[s] == k);
+        } else {
+          const T v = absT(r[s][k]);
+          const T kk = (v != v) ? (pos[s] == k ? infT<T>() : T(-1)) : v;
+          key[s] = cand ? kk : T(-2);
+        }
         best = key[s] > best ? key[s] : best;
       });
-      const T mx = group_allreduce<L>(best, [](T a, T b) { return a > b ? a : b; });
+      const K mx = group_allreduce<L>(best, [](K a, K b) { return a > b ? a : b; });
       int cpl = 0x7fffffff;
       static_for<0, R>([&](auto s) {
         const bool c = act[s] && pos[s] >= k && key[s] == mx;
