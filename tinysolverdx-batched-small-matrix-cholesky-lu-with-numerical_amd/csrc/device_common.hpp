@@ -212,34 +212,6 @@ __device__ __forceinline__ bool holds_max_key(T v, bool cand, bool nan_is_inf) {
 }
 
 
-// Smallest c over the NB lanes of each group, for lanes that hold no candidate
-// passing c = INT_MAX: the second half of the pivot search (the smallest
-// position among the lanes that hold the largest key).  Usually one lane per
-// group holds the maximum; when no group has two (the ballot has at most one
-// bit per group), that lane's c is read with readlane at the ballot's first
-// set bit (scalar work) instead of the lane steps of a reduction, and a group
-// with no such lane gets INT_MAX, as the reduction gives.  Ties take the
-// reduction.  Groups of 32 or 64 lanes (one or two per wave); smaller groups
-// always reduce.  Every lane of the wave must be active.
-template <int NB>
-__device__ __forceinline__ int group_min_sparse(int c, int g) {
-  if constexpr (NB == 32 || NB == 64) {
-    const uint64_t m = __builtin_amdgcn_ballot_w64(c != 0x7fffffff);
-    const unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
-    const bool single = NB == 64 ? (m & (m - 1)) == 0 : ((lo & (lo - 1)) | (hi & (hi - 1))) == 0;
-    if (single) {
-      if constexpr (NB == 64) {
-        return m ? __builtin_amdgcn_readlane(c, (int)__builtin_ctzll(m)) : 0x7fffffff;
-      } else {
-        const int c0 = lo ? __builtin_amdgcn_readlane(c, (int)__builtin_ctz(lo)) : 0x7fffffff;
-        const int c1 = hi ? __builtin_amdgcn_readlane(c, 32 + (int)__builtin_ctz(hi)) : 0x7fffffff;
-        return g ? c1 : c0;
-      }
-    }
-  }
-  return group_allreduce<NB>(c, [](int a, int b) { return a < b ? a : b; });
-}
-
 // Returns x unchanged but hides its value from the optimiser.  Used on the
 // lane index at the top of the right-hand-side chunk loops: otherwise LLVM
 // hoists the ~3*NB lane masks of the substitution passes (i == k, i < k, ...)

@@ -81,7 +81,7 @@ __device__ __forceinline__ void lu_emu(float (&r)[NB], int n, int i, int g, floa
     if (k < n) {
       const bool cand = act && pos >= k;
       const int cp = holds_max_key<NB>(__builtin_fabsf(r[k]), cand, pos == k) ? pos : 0x7fffffff;
-      const int p = group_min_sparse<NB>(cp, g);
+      const int p = group_allreduce<NB>(cp, [](int a, int b) { return a < b ? a : b; });
       if (i == k) mypiv = p;
       const bool isp = act && pos == p;
       pos = (pos == k) ? p : (isp ? k : pos);

@@ -105,7 +105,7 @@ __device__ __forceinline__ void lu_regs(T (&r)[NB], int nloop, int n, int i, int
     if (k < nloop) {
       const bool cand = act && pos >= k;
       const int cp = holds_max_key<NB>(absT(r[k]), cand, pos == k) ? pos : 0x7fffffff;
-      const int p = group_min_sparse<NB>(cp, g);
+      const int p = group_allreduce<NB>(cp, [](int a, int b) { return a < b ? a : b; });
       if (i == k) mypiv = p;
       const bool isp = act && pos == p;
       pos = (pos == k) ? p : (isp ? k : pos);
