@@ -60,6 +60,45 @@ def backward_error(A, X, B):
     return (num / den).amax(dim=1)
 
 
+def kappa_decades(kappa, X64, tiers):
+    """Per decade of kappa: the matrix count and, for every solve in tiers
+    (name -> (X, A, B, stat or None, tier or None)), the largest forward error
+    against the plain float64 solve X64 (max over right-hand sides of the
+    relative 2-norm difference), the largest backward error, the share of
+    refinement fallbacks (stat -1 where the routed choice was not float64)
+    and, given tier, the share the network sent to float64."""
+    import torch
+    dec = torch.floor(torch.log10(kappa.to(torch.float64))).to(torch.int64)
+    ref = torch.linalg.vector_norm(X64, dim=1)
+    out = {}
+    stats = {}
+    for name, (X, A, B, st, tr) in tiers.items():
+        fw = (torch.linalg.vector_norm(X - X64, dim=1) / ref).amax(dim=1)
+        bw = backward_error(A, X, B)
+        fb = None
+        if st is not None:
+            fb = st < 0
+            if tr is not None:
+                fb = fb & (tr != 2)
+        stats[name] = (fw, bw, fb, tr)
+    for d in range(int(dec.min().item()), int(dec.max().item()) + 1):
+        sel = dec == d
+        cnt = int(sel.sum().item())
+        if cnt == 0:
+            continue
+        row = {"matrices": cnt}
+        for name, (fw, bw, fb, tr) in stats.items():
+            r = {"max_fwd_err_vs_fp64": float(f"{fw[sel].max().item():.3e}"),
+                 "max_bwd_err": float(f"{bw[sel].max().item():.3e}")}
+            if fb is not None:
+                r["fallback_rate"] = round(int((fb & sel).sum().item()) / cnt, 5)
+            if tr is not None:
+                r["fp64_tier_share"] = round(int(((tr == 2) & sel).sum().item()) / cnt, 5)
+            row[name] = r
+        out[f"1e{d}-1e{d + 1}"] = row
+    return out
+
+
 def time_steps(fn, steps, warmup):
     """Warmup, then exactly `steps` calls bracketed by device synchronisation;
     returns (wall seconds, HIP-event seconds) for the K calls."""
@@ -246,6 +285,11 @@ def run_posv(a):
     if ok_r != batch:
         log("[bench] WARNING: not every routed answer meets the 1e-14 bar")
 
+    # error against the float64 solve and failure rate per kappa decade
+    decades = kappa_decades(kappa, X, {"fixed_tier": (Xm, A, B, stat, None),
+                                       "routed": (Xr, A, B, stat_r, tier)})
+    log(f"[bench] per kappa decade: {json.dumps(decades)}")
+
     # algorithmic bytes per matrix: read A (8n^2), read B and write X (8n*nrhs each), info (4)
     bytes_per = 8 * n * n + 2 * 8 * n * nrhs + 4
     achieved = bytes_per * batch / routed_kern / 1e9
@@ -282,6 +326,7 @@ def run_posv(a):
                    "plain_fp64_solves_per_s": round(plain_rate, 1),
                    "plain_fp64_bwd_err_le_1e-14": f"{ok}/{batch}",
                    "fixed_tier_over_plain": round(mixed_rate / plain_rate, 3),
+                   "per_kappa_decade": decades,
                    # the reference's convention (its bench.py): posv = n^3/3 + 2 n^2 nrhs
                    "gflops_routed_fixed_plain": [round(r * (n ** 3 / 3 + 2 * n * n * nrhs) / 1e9, 1)
                                                  for r in (routed_rate, mixed_rate, plain_rate)]},
