@@ -47,6 +47,22 @@ __device__ __forceinline__ void static_for_rev(F&& f) {
   }
 }
 
+// f(ic<j>) for the columns j in [B, E), 16 at a time; with CH, a run of 16
+// columns that starts at or past `bound` (wave-uniform) is skipped with one
+// scalar branch.
+template <int B, int E, bool CH, class F>
+__device__ __forceinline__ void static_for_cols(int bound, F&& f) {
+  if constexpr (!CH) {
+    static_for<B, E>(f);
+  } else {
+    static_for<B / 16, (E + 15) / 16>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      constexpr int lo = 16 * c > B ? 16 * c : B, hi = 16 * c + 16 < E ? 16 * c + 16 : E;
+      if (16 * c < bound) static_for<lo, hi>(f);
+    });
+  }
+}
+
 // ------------------------------------------------------------- bit helpers
 __device__ __forceinline__ int lo32(double x) { return (int)(__builtin_bit_cast(unsigned long long, x) & 0xffffffffull); }
 __device__ __forceinline__ int hi32(double x) { return (int)(__builtin_bit_cast(unsigned long long, x) >> 32); }

@@ -84,10 +84,15 @@ __device__ __forceinline__ T diag_inv(const T (&r)[NB], int i) {
 // Rows never move between lanes: lane i keeps original row i and tracks its
 // current position `pos`.  On exit row `pos` of the packed L\U factor is in r,
 // lane k holds piv[k] in mypiv.  rowb: per-wave LDS, 2*64 elements.
-// nloop / n as for chol_regs.
-template <class T, int NB, bool ZTILE>
+// nloop / n as for chol_regs.  COLS: columns at or past nloop (rounded up to
+// 16) are neither copied nor updated.  Those columns are zero padding of every
+// matrix of the group (nloop >= its n), which no later step reads for its own
+// rows and which is never stored, so the results do not change; ragged tasks,
+// whose nloop is the largest n of the task, skip that work.
+template <class T, int NB, bool ZTILE, bool COLS = false>
 __device__ __forceinline__ void lu_regs(T (&r)[NB], int nloop, int n, int i, int g, T* rowb, int& pos, int& mypiv,
                                         int& info) {
+  const int bound = COLS ? __builtin_amdgcn_readfirstlane(nloop) : NB;
   pos = i;
   mypiv = 0;
   info = 0;
@@ -115,7 +120,7 @@ __device__ __forceinline__ void lu_regs(T (&r)[NB], int nloop, int n, int i, int
         // lets the registers go out as the pairs they are held in (the column
         // left of k is never read).  Either start gave the fewer register moves
         // for its type.
-        static_for<(sizeof(T) == 8 ? k / 2 * 2 : k), NB>([&](auto jc) { rb[jc] = r[jc]; });
+        static_for_cols<(sizeof(T) == 8 ? k / 2 * 2 : k), NB, COLS>(bound, [&](auto jc) { rb[jc] = r[jc]; });
       }
       wave_sync();
       const T pk = rb[k];
@@ -132,14 +137,14 @@ __device__ __forceinline__ void lu_regs(T (&r)[NB], int nloop, int n, int i, int
         const T q = r[k] / (zero ? T(1) : pk);
         r[k] = upd ? q : r[k];
         const T f = upd ? q : T(0);
-        static_for<k + 1, NB>([&](auto jc) {
+        static_for_cols<k + 1, NB, COLS>(bound, [&](auto jc) {
           constexpr int j = decltype(jc)::value;
           r[j] = fmaT(-f, y[j], r[j]);
         });
       } else if (upd) {
         const T f = r[k] / (zero ? T(1) : pk);
         r[k] = f;
-        static_for<k + 1, NB>([&](auto jc) {
+        static_for_cols<k + 1, NB, COLS>(bound, [&](auto jc) {
           constexpr int j = decltype(jc)::value;
           r[j] = fmaT(-f, rb[j], r[j]);
         });

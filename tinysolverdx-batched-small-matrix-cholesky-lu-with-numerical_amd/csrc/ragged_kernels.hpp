@@ -118,7 +118,7 @@ __device__ __forceinline__ void ragged_chol(const RaggedArgs<T>& a, const RTask&
     const int row = q + L * s;
     load_row<T, N>(r[s], x.A + (int64_t)row * x.n, x.n, x.live && row < x.n, x.vec);
   });
-  const int inf = chol_rows<T, L, R>(r, t.nmax, x.n, q, g, col);
+  const int inf = chol_rows<T, L, R, NoSlotHook, (N > 32)>(r, t.nmax, x.n, q, g, col);
   static_for<0, R>([&](auto s) {
     const int row = q + L * s;
     if (x.live && row < x.n) store_row<T, N>(r[s], x.A + (int64_t)row * x.n, x.n, x.vec, true, row);
@@ -135,7 +135,7 @@ __device__ __forceinline__ void ragged_lu(const RaggedArgs<T>& a, const RTask& t
   T r[NB];
   load_row<T, NB>(r, x.A + (int64_t)i * x.n, x.n, io, x.vec);
   int pos, mypiv, inf;
-  lu_regs<T, NB, ZTILE>(r, t.nmax, x.n, i, g, rowb, pos, mypiv, inf);
+  lu_regs<T, NB, ZTILE, (NB > 32)>(r, t.nmax, x.n, i, g, rowb, pos, mypiv, inf);
   if (io) {
     store_row<T, NB>(r, x.A + (int64_t)pos * x.n, x.n, x.vec, false, 0);
     if (a.piv) a.piv[a.poff[x.m] + i] = mypiv;
@@ -155,7 +155,7 @@ __device__ __forceinline__ void ragged_lu_rows(const RaggedArgs<T>& a, const RTa
     load_row<T, N>(r[s], x.A + (int64_t)row * x.n, x.n, x.live && row < x.n, x.vec);
   });
   int pos[R], mypiv[R], inf;
-  lu_rows<T, L, R, ZTILE>(r, t.nmax, x.n, q, g, rowb, pos, mypiv, inf);
+  lu_rows<T, L, R, ZTILE, (L * R > 32)>(r, t.nmax, x.n, q, g, rowb, pos, mypiv, inf);
   static_for<0, R>([&](auto s) {
     const int row = q + L * s;
     if (x.live && row < x.n) {

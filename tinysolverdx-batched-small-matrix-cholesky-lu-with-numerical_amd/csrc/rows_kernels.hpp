@@ -73,14 +73,19 @@ struct ColumnDivider<double, R> {
 // done(ic<s>) is called as soon as slot s < R - 1 holds final rows (after the
 // pivot of row L*s + L - 1), so that a caller can store them while the
 // factorization goes on; later steps leave those rows' lower parts alone.
+// COLS: columns at or past nloop (rounded up to 16) are not updated: they are
+// zero padding of every matrix of the group (nloop >= its n), never read for
+// a row below nloop and never stored, so the results do not change (ragged
+// tasks, whose nloop is the largest n of the task).
 struct NoSlotHook {
   template <class S>
   __device__ __forceinline__ void operator()(S) const {}
 };
-template <class T, int L, int R, class Done = NoSlotHook>
+template <class T, int L, int R, class Done = NoSlotHook, bool COLS = false>
 __device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int nloop, int n, int q, int g, T* col,
                                          Done done = Done{}) {
   constexpr int N = L * R;
+  const int bound = COLS ? __builtin_amdgcn_readfirstlane(nloop) : N;
   int info = 0;
   auto pivot = [&](auto kc) {
     constexpr int k = decltype(kc)::value;
@@ -108,7 +113,7 @@ __device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int nloop, int n, int
       constexpr int SA = (k + 1) / L;  // first slot with a row > k
       static_for<SA, R>([&](auto s) { r[s][k + 1] = fmaT(-r[s][k], cb[k + 1], r[s][k + 1]); });
       pivot(ic<k + 1>{});
-      static_for<k + 2, N>([&](auto jc) {
+      static_for_cols<k + 2, N, COLS>(bound, [&](auto jc) {
         constexpr int j = decltype(jc)::value;
         const T c = cb[j];
         static_for<SA, R>([&](auto s) {
@@ -136,11 +141,12 @@ __device__ __forceinline__ int chol_rows(T (&r)[R][L * R], int nloop, int n, int
 // What R rows per lane buy: the pivot search reduces over the lane's R rows
 // first and over L lanes after, and one refined reciprocal of the pivot
 // gives the lane's R quotients (ColumnDivider: correctly rounded).
-// rowb: per-wave LDS, 2 * 64 * R elements.
-template <class T, int L, int R, bool ZTILE>
+// rowb: per-wave LDS, 2 * 64 * R elements.  COLS as for lu_regs.
+template <class T, int L, int R, bool ZTILE, bool COLS = false>
 __device__ __forceinline__ void lu_rows(T (&r)[R][L * R], int nloop, int n, int q, int g, T* rowb, int (&pos)[R],
                                         int (&mypiv)[R], int& info) {
   constexpr int N = L * R;
+  const int bound = COLS ? __builtin_amdgcn_readfirstlane(nloop) : N;
   info = 0;
   bool act[R];
   static_for<0, R>([&](auto s) {
@@ -180,7 +186,7 @@ __device__ __forceinline__ void lu_rows(T (&r)[R][L * R], int nloop, int n, int 
       });
       T* rb = rowb + (k & 1) * (kWave * R) + g * N;
       static_for<0, R>([&](auto s) {
-        if (act[s] && pos[s] == k) static_for<k, N>([&](auto jc) { rb[jc] = r[s][jc]; });
+        if (act[s] && pos[s] == k) static_for_cols<k, N, COLS>(bound, [&](auto jc) { rb[jc] = r[s][jc]; });
       });
       wave_sync();
       const T pk = rb[k];
@@ -193,7 +199,7 @@ __device__ __forceinline__ void lu_rows(T (&r)[R][L * R], int nloop, int n, int 
         static_for<0, R>([&](auto s) {
           if (act[s] && pos[s] > k) {
             r[s][k] = f[s];
-            static_for<k + 1, N>([&](auto jc) {
+            static_for_cols<k + 1, N, COLS>(bound, [&](auto jc) {
               constexpr int j = decltype(jc)::value;
               r[s][j] = fmaT(-f[s], rb[j], r[s][j]);
             });
