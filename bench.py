@@ -225,6 +225,11 @@ def run_posv(a):
     work = device.posv_ir_workspace(batch)
     from tinysolverdx import router as rt
     R = rt.default()  # the shipped router (trained on the GPU, tinysolverdx/router_default.npy)
+    net = rt.Router(R.params.cpu().numpy())  # the same network at every size, for comparison
+    net_used = rt.bucket(n) not in (R.direct | R.fp32_direct)
+    Xn = torch.empty_like(B)
+    stat_n = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    tier_n = torch.zeros(batch, dtype=torch.int32, device="cuda")
     Xr = torch.empty_like(B)
     info_r = torch.zeros(batch, dtype=torch.int32, device="cuda")
     stat_r = torch.zeros(batch, dtype=torch.int32, device="cuda")
@@ -240,16 +245,23 @@ def run_posv(a):
     def mixed():
         device.posv_ir(A, B, Xm, factor=a.factor, max_iter=10, tol=1e-14, info=info_m, stat=stat, work=work)
 
-    # ---- routed: the router picks the tier of every matrix inside the solve
-    # kernel; float64 residual and fallback as above
+    # ---- routed: the shipped router's policy for this size (the network
+    # picking each matrix's tier inside the solve kernel, or one tier for every
+    # matrix where the router's timing found that faster); float64 residual
+    # and fallback as above
     def routed():
         R.posv(A, B, Xr, info=info_r, stat=stat_r, tier=tier, work=work_r)
 
-    for fn in (plain, mixed, routed):  # every path once before anything is timed (clocks, code objects)
+    # ---- the network at this size whatever the policy (for comparison)
+    def network():
+        net.posv(A, B, Xn, stat=stat_n, tier=tier_n, work=work_r)
+
+    for fn in (plain, mixed, routed, network):  # every path once before anything is timed (clocks, code objects)
         fn()
     wall, ev = time_steps(plain, a.steps, a.warmup)
     wall_m, ev_m = time_steps(mixed, a.steps, a.warmup)
     wall_r, ev_r = time_steps(routed, a.steps, a.warmup)
+    wall_n, ev_n = time_steps(network, a.steps, a.warmup)
 
     bw = backward_error(A, X, B)
     fails = int((info != 0).sum().item())
@@ -284,27 +296,42 @@ def run_posv(a):
         f"max {bw_r.max().item():.2e}")
     if ok_r != batch:
         log("[bench] WARNING: not every routed answer meets the 1e-14 bar")
+    policy = "network" if net_used else ("fp32+IR for every matrix" if rt.bucket(n) in R.fp32_direct
+                                          else "fp64 for every matrix")
+    bw_n = backward_error(A, Xn, B)
+    ok_n = int((bw_n <= 1e-14).sum().item())
+    net_rate = batch * a.steps / wall_n
+    tiers_n = np.bincount(tier_n.cpu().numpy(), minlength=3)[:3].tolist()
+    log(f"[bench] shipped policy at n={n}: {policy}; the network at n={n}: {ev_n / a.steps * 1e3:.3f} ms/call, "
+        f"{net_rate / 1e6:.2f} M solves/s, tiers {tiers_n}, bwd err <= 1e-14: {ok_n}/{batch}")
 
     # error against the float64 solve and failure rate per kappa decade
     decades = kappa_decades(kappa, X, {"fixed_tier": (Xm, A, B, stat, None),
-                                       "routed": (Xr, A, B, stat_r, tier)})
+                                       "routed": (Xr, A, B, stat_r, tier),
+                                       "network": (Xn, A, B, stat_n, tier_n)})
     log(f"[bench] per kappa decade: {json.dumps(decades)}")
 
     # algorithmic bytes per matrix: read A (8n^2), read B and write X (8n*nrhs each), info (4)
     bytes_per = 8 * n * n + 2 * 8 * n * nrhs + 4
     achieved = bytes_per * batch / routed_kern / 1e9
     traffic = None
+    if net_used:
+        kname = "k_posv_mx_routed<4>"
+        kdesc = ("k_posv_mx_routed<4> (features + network + bf16/fp32 factor on MFMA + float64 refinement in one "
+                 "kernel; the event window also holds the fallback-count reset and the float64 tier's launch over "
+                 "the matrices routed to it, k_posv_mx64<4>)")
+    else:
+        kname = "k_posv_mx<4,F32>" if rt.bucket(n) in R.fp32_direct else "k_posv_mx64<4,1>"
+        kdesc = (f"{kname} (the shipped router's policy at n={n}: {policy}; the event window also holds the "
+                 "fallback-count reset and the float64 tier's launch over the fallback list)")
     try:  # HBM bytes per launch measured by rocprofv3 for this workload (profiles/traffic.json)
         if batch == 131072 and nrhs == 1:
-            traffic = json.loads((ROOT / "profiles" / "traffic.json").read_text())[
-                "posv_n64_b131072_nrhs1"]["k_posv_mx_routed<4>"]
+            traffic = json.loads((ROOT / "profiles" / "traffic.json").read_text())["posv_n64_b131072_nrhs1"][kname]
     except (OSError, KeyError, ValueError):
         traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_posv_mx_routed<4> (features + network + bf16/fp32 factor on MFMA + float64 refinement "
-                          "in one kernel; the event window also holds the fallback-count reset and the float64 "
-                          "tier's launch over the matrices routed to it, k_posv_mx64<4>)",
+                "kernel": kdesc,
                 "bytes_per_matrix": bytes_per,
                 "fixed_tier_kernel": {"kernel": f"k_posv_mx<4> ({a.factor})", "ms": round(mixed_kern * 1e3, 4),
                                       "achieved": round(bytes_per * batch / mixed_kern / 1e9, 1)},
@@ -316,11 +343,19 @@ def run_posv(a):
         "warmup": a.warmup, "ms_per_step": round(wall_r / a.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"posv n={n} SPD kappa log-uniform [1e1,1e6], batch={batch}, nrhs={nrhs}, fp64 inputs",
-                   "tier": "routed: learned router per matrix over {bf16+IR, fp32+IR, fp64}; refinement <=10 "
-                           "corrections to tol 1e-14 with fp64 fallback",
+                   "tier": "routed: the shipped router (tinysolverdx.router.default): per size bucket the network "
+                           "over {bf16+IR, fp32+IR, fp64} per matrix, or one tier for every matrix where its timing "
+                           "on the training distribution found that faster; refinement <=10 corrections to tol "
+                           "1e-14 with fp64 fallback",
+                   "routed_policy_at_n": policy,
+                   "routed_policy_ns_per_matrix": R.policy_times.get(str(rt.bucket(n))),
                    "routed_tiers_bf16_fp32_fp64": tiers, "routed_refinement_fallbacks": fallback_r,
                    "routed_bwd_err_le_1e-14": f"{ok_r}/{batch}",
                    "routed_over_plain": round(routed_rate / plain_rate, 3),
+                   "network_solves_per_s": round(net_rate, 1),
+                   "network_over_plain": round(net_rate / plain_rate, 3),
+                   "network_tiers_bf16_fp32_fp64": tiers_n,
+                   "network_bwd_err_le_1e-14": f"{ok_n}/{batch}",
                    "fixed_tier": a.factor, "fixed_tier_solves_per_s": round(mixed_rate, 1),
                    "bwd_err_le_1e-14": f"{ok_m}/{batch}", "fp64_fallback": fallback,
                    "plain_fp64_solves_per_s": round(plain_rate, 1),
@@ -697,17 +732,18 @@ def run_router(a):
     hist = R.fit(feat, label, epochs=60, batch=4096, lr=3e-3, seed=3)
     train_acc = float((R.predict(feat) == label).float().mean().item())
     shift = R.calibrate(rt.training_set.costs[64])  # expected-cost decision between fp32 and float64
-    direct, dtimes = rt.choose_direct(R)  # size buckets where the network does not pay (timed, fresh seed)
-    R.direct = frozenset(direct)
+    # per-size policy: the network or one tier for every matrix, whichever is fastest (timed, fresh seed)
+    direct, direct32, dtimes = rt.choose_direct(R)
+    R.direct, R.fp32_direct = frozenset(direct), frozenset(direct32)
+    net = rt.Router(R.params.cpu().numpy())  # the same network at every size, for comparison
     log(f"[bench] router trained in {time.perf_counter() - t0:.1f}s: loss {hist[0]:.3f} -> {hist[-1]:.3f}, "
         f"labels {np.bincount(label.cpu().numpy(), minlength=3).tolist()}, train accuracy {train_acc:.3f}, "
-        f"fp64 logit shift {shift:.3f}, "
-        f"direct float64 buckets {direct} (routed vs fp64 ns/matrix "
+        f"fp64 logit shift {shift:.3f}, direct buckets float64 {direct} fp32 {direct32} (ns/matrix "
         f"{ {n: {k: round(v * 1e9, 2) for k, v in t.items()} for n, t in dtimes.items()} })")
     sizes = (16, 32, 64)
     per = total // 3
-    ms = {"routed": 0.0, "unrouted": 0.0, "fp64": 0.0}
-    ok = {"routed": 0, "unrouted": 0, "fp64": 0}
+    ms = {"routed": 0.0, "network": 0.0, "unrouted": 0.0, "fp64": 0.0}
+    ok = {"routed": 0, "network": 0, "unrouted": 0, "fp64": 0}
     tiers = np.zeros(3, np.int64)
     wasted = {"routed": 0, "unrouted": 0}
     count = 0
@@ -723,6 +759,7 @@ def run_router(a):
                          device="cuda")
         wi = device.posv_ir_workspace(per)
         runs = {"routed": lambda: R.posv(A, B, X, info=info, stat=stat, tier=tier, work=wr),
+                "network": lambda: net.posv(A, B, X, info=info, stat=stat, work=wr),
                 "unrouted": lambda: device.posv_ir(A, B, X, factor="fp32", info=info, stat=stat, work=wi),
                 "fp64": lambda: device.posv_solve(A, B, X, info)}
         for name, fn in runs.items():
@@ -741,12 +778,15 @@ def run_router(a):
         log(f"[bench] router n={n}: cumulative ms {ms}")
     rate = {k: count / (v * 1e-3) for k, v in ms.items()}
     es_bytes = sum(8 * n * n + 2 * 8 * n + 4 for n in sizes) * per
-    log(f"[bench] router: routed {rate['routed'] / 1e6:.2f} M/s, un-routed fp32 {rate['unrouted'] / 1e6:.2f} M/s, "
-        f"fp64 {rate['fp64'] / 1e6:.2f} M/s, tiers {tiers.tolist()}, wasted attempts {wasted}")
+    log(f"[bench] router: routed {rate['routed'] / 1e6:.2f} M/s, network at every size {rate['network'] / 1e6:.2f} M/s, "
+        f"un-routed fp32 {rate['unrouted'] / 1e6:.2f} M/s, fp64 {rate['fp64'] / 1e6:.2f} M/s, tiers {tiers.tolist()}, "
+        f"wasted attempts {wasted}")
     cfg = {"workload": f"routed posv, {count} SPD fp64 matrices, n uniform on {{16,32,64}}, kappa log-uniform "
                        "[1,1e12], nrhs=1; router trained in this run on the shipped training mix (tinysolverdx.router.TRAINING_MIX), separately seeded",
            "train_loss_first_last": [round(hist[0], 4), round(hist[-1], 4)], "train_accuracy": round(train_acc, 4),
-           "direct_float64_buckets": direct, "fp64_logit_shift": round(shift, 4),
+           "direct_float64_buckets": direct, "direct_fp32_buckets": direct32, "fp64_logit_shift": round(shift, 4),
+           "policy_ns_per_matrix": {n: {k: round(v * 1e9, 2) for k, v in t.items()} for n, t in dtimes.items()},
+           "network_at_every_size_solves_per_s": round(rate["network"], 1),
            "train_labels_bf16_fp32_fp64": np.bincount(label.cpu().numpy(), minlength=3).tolist(),
            "routed_tiers_bf16_fp32_fp64": tiers.tolist(),
            "unrouted_fp32_solves_per_s": round(rate["unrouted"], 1), "plain_fp64_solves_per_s": round(rate["fp64"], 1),

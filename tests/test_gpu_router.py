@@ -196,7 +196,7 @@ def test_router_gain_measured(tsdx):
     import torch
     from tinysolverdx import _lib, device, matrices, router
     r = router.default()
-    shipped = r.direct
+    shipped = (r.direct, r.fp32_direct)
     cnt = 16384
     times = {k: {} for k in ("routed", "network", "fp32", "fp64", "rule")}
     for n in (16, 32, 64):
@@ -207,15 +207,15 @@ def test_router_gain_measured(tsdx):
         wi = device.posv_ir_workspace(cnt)
         wr = torch.empty(_lib.lib().tsdx_hip_posv_routed_workspace_size(cnt), dtype=torch.uint8, device="cuda")
         fp64 = "fp64_block" if n > 32 else "fp64"
-        r.direct = shipped
+        r.direct, r.fp32_direct = shipped
         times["routed"][n] = _best_ms(lambda: r.posv(A, B, X, work=wr))
         Xr = X.clone()
-        r.direct = frozenset()
+        r.direct, r.fp32_direct = frozenset(), frozenset()
         times["network"][n] = _best_ms(lambda: r.posv(A, B, X, work=wr))
-        r.direct = shipped
+        r.direct, r.fp32_direct = shipped
         times["fp32"][n] = _best_ms(lambda: device.posv_ir(A, B, X, factor="fp32", work=wi))
         times["fp64"][n] = _best_ms(lambda: device.posv_ir(A, B, X, factor=fp64, work=wi))
-        times["rule"][n] = times["fp64"][n] if router.bucket(n) in shipped else times["fp32"][n]
+        times["rule"][n] = times["fp64"][n] if router.bucket(n) in shipped[0] else times["fp32"][n]
         sel = torch.arange(0, cnt, 64, device="cuda")  # the bar, on a sample (the CPU metric is float64 numpy)
         assert _bwd(tsdx, A[sel].cpu().numpy(), Xr[sel].cpu().numpy(), B[sel].cpu().numpy()).max() <= 1e-14, n
     total = {k: sum(v.values()) for k, v in times.items()}
@@ -240,7 +240,7 @@ def test_fused_features_match_standalone_kernel():
         feat = torch.zeros((24, 12), dtype=torch.float32, device="cuda")
         tier = torch.full((24,), -1, dtype=torch.int32, device="cuda")
         r = router.default()
-        r.direct = frozenset()
+        r.direct, r.fp32_direct = frozenset(), frozenset()
         r.posv(dA, B, torch.zeros_like(B), feat=feat, tier=tier)
         got = feat.cpu().numpy()
         # the routed kernel's choice is the inference path's on its features
@@ -271,5 +271,40 @@ def test_direct_bucket_takes_the_float64_tier(tsdx, n):
     info, stat, tier = info.cpu().numpy(), stat.cpu().numpy(), tier.cpu().numpy()
     assert (tier == 2).all() and (stat == -1).all()
     assert info[5] == 1 and (np.delete(info, 5) == 0).all()
+    ok = info == 0
+    assert _bwd(tsdx, A[ok], X.cpu().numpy()[ok], B[ok]).max() <= 1e-14
+
+
+@pytest.mark.parametrize("n", [64, 40])
+def test_fp32_direct_bucket_takes_the_refinement_tier(tsdx, n):
+    """A size bucket marked fp32 direct skips the network: every matrix takes
+    the fp32 refinement tier (tier 1), whose float64 fallback still solves
+    what refinement cannot (stat -1, among them a matrix that is not SPD,
+    with the plain posv's info); every other answer meets the bar and equals
+    tsdx_hip_posv_ir_f64's with the fp32 factor bit for bit."""
+    import torch
+    from tinysolverdx import device, router
+    rng = np.random.default_rng(31 + n)
+    A, _ = _spd(rng, 40, n, 0, 12)
+    A[5] = -np.eye(n)
+    B = rng.standard_normal((40, n, 1))
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    r = router.Router(router.init_params(0), fp32_direct=[64])
+    X = torch.zeros_like(dB)
+    info = torch.full((40,), -9, dtype=torch.int32, device="cuda")
+    stat = torch.full((40,), -9, dtype=torch.int32, device="cuda")
+    tier = torch.full((40,), -9, dtype=torch.int32, device="cuda")
+    r.posv(dA, dB, X, info=info, stat=stat, tier=tier)
+    Xw = torch.zeros_like(dB)
+    info_w = torch.full((40,), -9, dtype=torch.int32, device="cuda")
+    stat_w = torch.full((40,), -9, dtype=torch.int32, device="cuda")
+    device.posv_ir(dA, dB, Xw, factor="fp32", info=info_w, stat=stat_w)
+    torch.cuda.synchronize()
+    assert (tier.cpu().numpy() == 1).all()
+    assert np.array_equal(info.cpu().numpy(), info_w.cpu().numpy())
+    assert np.array_equal(stat.cpu().numpy(), stat_w.cpu().numpy())
+    assert np.array_equal(X.cpu().numpy().view(np.uint64), Xw.cpu().numpy().view(np.uint64))
+    info = info.cpu().numpy()
+    assert info[5] == 1 and stat.cpu().numpy()[5] == -1
     ok = info == 0
     assert _bwd(tsdx, A[ok], X.cpu().numpy()[ok], B[ok]).max() <= 1e-14

@@ -26,6 +26,20 @@ def test_direct_buckets_validated():
     from tinysolverdx import router
     with pytest.raises(ValueError):
         router.Router(router.init_params(0), device="cpu", direct=[24])
+    with pytest.raises(ValueError):
+        router.Router(router.init_params(0), device="cpu", fp32_direct=[48])
+    with pytest.raises(ValueError):  # one policy per bucket
+        router.Router(router.init_params(0), device="cpu", direct=[64], fp32_direct=[64])
+
+
+def test_fp32_direct_buckets_round_trip(tmp_path):
+    from tinysolverdx import router
+    r = router.Router(router.init_params(4), device="cpu", direct=[16], fp32_direct=[64, 32])
+    path = tmp_path / "r.npy"
+    r.save(path, {"policy_ns_per_matrix": {"64": {"routed": 2.0, "fp64": 3.0, "fp32": 1.0}}})
+    q = router.Router.load(path, device="cpu")
+    assert q.direct == {16} and q.fp32_direct == {32, 64}
+    assert q.policy_times["64"]["fp32"] == 1.0
 
 
 def test_shipped_router_metadata():
@@ -35,7 +49,15 @@ def test_shipped_router_metadata():
     from tinysolverdx import router
     meta = json.loads(router.DEFAULT_PARAMS.with_suffix(".json").read_text())
     assert set(meta["direct_buckets"]) <= set(router.BUCKETS)
+    assert set(meta.get("fp32_direct_buckets", [])) <= set(router.BUCKETS)
+    assert not set(meta["direct_buckets"]) & set(meta.get("fp32_direct_buckets", []))
     assert "cost_ns_per_matrix" in meta["training"]
+    # every direct bucket is the fastest of the three timed policies for its size
+    for n, t in meta["training"].get("policy_ns_per_matrix", {}).items():
+        best = min(t, key=t.get)
+        b = router.bucket(int(n))
+        assert (b in meta["direct_buckets"]) == (best == "fp64"), (n, t)
+        assert (b in meta.get("fp32_direct_buckets", [])) == (best == "fp32"), (n, t)
 
 
 def test_calibrate_shifts_the_float64_logit():

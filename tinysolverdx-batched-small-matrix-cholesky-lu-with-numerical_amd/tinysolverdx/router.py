@@ -18,13 +18,15 @@ The routed solve (Router.posv) never trusts the network for accuracy: every
 answer is judged by the float64 residual, refinement that stalls falls back
 to the float64 solve, exactly as tsdx_hip_posv_ir_f64.
 
-A router also carries the sizes it solves directly in float64 without
-consulting the network (Router.direct, size buckets 4/8/16/32/64): after
-training, train_default times the routed solve against the float64 tier for
-every matrix on separately seeded matrices of the training distribution, and
-a bucket where routing does not pay (small n: the features and the network
-cost more than the whole float64 solve there) is marked direct.  The choice is
-stored next to the parameters (router_default.json).
+A router also carries the size buckets (4/8/16/32/64) it solves with one
+tier for every matrix, without consulting the network: Router.direct (the
+float64 tier) and Router.fp32_direct (fp32 factor + refinement, float64
+fallback).  After training, choose_direct times, per size, the routed solve
+against both fixed tiers on separately seeded matrices of the training
+distribution and keeps the fastest policy: a bucket where a fixed tier beats
+the network (its features and its wrong guesses cost more than its right
+ones save) is marked direct for that tier.  The choice is stored next to the
+parameters (router_default.json).
 """
 import numpy as np
 
@@ -119,15 +121,16 @@ def bucket(n):
 
 
 def choose_direct(r, sizes=(16, 32, 64), count=65536, kmax=1e12, seed=20261210, reps=3):
-    """Size buckets where Router.posv is faster solving every matrix in
-    float64 (the direct path) than routing, timed both ways on `count` SPD
-    matrices per size with kappa log-uniform on [1, kmax] (the training
-    distribution, fresh seed).  Returns (direct buckets, {n: {"routed": s,
-    "fp64": s}} per matrix)."""
+    """Per size, the fastest of three policies for Router.posv, timed on
+    `count` SPD matrices with kappa log-uniform on [1, kmax] (the training
+    distribution, fresh seed): the network, the float64 tier for every matrix,
+    the fp32 refinement tier for every matrix (float64 fallback included).
+    Returns (float64 buckets, fp32 buckets, {n: {"routed": s, "fp64": s,
+    "fp32": s}} per matrix)."""
     import torch
     from . import matrices
-    direct, times = [], {}
-    saved = r.direct
+    fp64, fp32, times = [], [], {}
+    saved = (r.direct, r.fp32_direct)
     try:
         for n in sizes:
             A, _ = matrices.spd_device(n, count, 1.0, kmax, seed + n)
@@ -135,17 +138,20 @@ def choose_direct(r, sizes=(16, 32, 64), count=65536, kmax=1e12, seed=20261210, 
                             dtype=torch.float64)
             X = torch.empty_like(B)
             wr = torch.empty(_lib.lib().tsdx_hip_posv_routed_workspace_size(count), dtype=torch.uint8, device="cuda")
-            r.direct = frozenset()
-            t_r = _timed(lambda: r.posv(A, B, X, work=wr), reps) / count
-            r.direct = frozenset([bucket(n)])
-            t_d = _timed(lambda: r.posv(A, B, X, work=wr), reps) / count
-            times[n] = {"routed": t_r, "fp64": t_d}
-            if t_d <= t_r:
-                direct.append(bucket(n))
+            t = {}
+            for name, d64, d32 in (("routed", (), ()), ("fp64", (bucket(n),), ()), ("fp32", (), (bucket(n),))):
+                r.direct, r.fp32_direct = frozenset(d64), frozenset(d32)
+                t[name] = _timed(lambda: r.posv(A, B, X, work=wr), reps) / count
+            times[n] = t
+            best = min(("routed", "fp64", "fp32"), key=lambda k: t[k])
+            if best == "fp64":
+                fp64.append(bucket(n))
+            elif best == "fp32":
+                fp32.append(bucket(n))
             del A, B, X
     finally:
-        r.direct = saved
-    return sorted(direct), times
+        r.direct, r.fp32_direct = saved
+    return sorted(fp64), sorted(fp32), times
 
 
 # Training mix of the shipped router: per size, (count, kappa_min, kappa_max)
@@ -190,16 +196,22 @@ def train_default(path=DEFAULT_PARAMS, mix=None, epochs=60, seed=20261207):
     hist = r.fit(feat, label, epochs=epochs, batch=4096, lr=3e-3, seed=seed)
     acc = float((r.predict(feat) == label).float().mean().item())
     shift = r.calibrate(training_set.costs[max(training_set.costs)])  # the largest size: where the network runs
-    direct, times = choose_direct(r)
-    r.direct = frozenset(direct)
     counts = np.bincount(label.cpu().numpy(), minlength=3).tolist()
     costs = {n: {t: round(c * 1e9, 3) for t, c in v.items()} for n, v in training_set.costs.items()}
     report = {"loss_first": hist[0], "loss_last": hist[-1], "label_counts": counts, "train_accuracy": acc,
-              "fp64_logit_shift": round(shift, 4), "cost_ns_per_matrix": costs, "direct_buckets": direct,
-              "routed_vs_fp64_ns_per_matrix": {n: {k: round(v * 1e9, 3) for k, v in t.items()}
-                                               for n, t in times.items()}}
+              "fp64_logit_shift": round(shift, 4), "cost_ns_per_matrix": costs}
+    report.update(direct_policy(r))
     r.save(path, report)
     return report
+
+
+def direct_policy(r):
+    """Runs choose_direct, sets r.direct / r.fp32_direct and returns the
+    report entries of the choice."""
+    d64, d32, times = choose_direct(r)
+    r.direct, r.fp32_direct = frozenset(d64), frozenset(d32)
+    return {"direct_buckets": d64, "fp32_direct_buckets": d32,
+            "policy_ns_per_matrix": {n: {k: round(v * 1e9, 3) for k, v in t.items()} for n, t in times.items()}}
 
 
 def default():
@@ -223,16 +235,20 @@ def init_params(seed=0):
 class Router:
     """Parameters live in a float32 device tensor of PARAMS entries."""
 
-    def __init__(self, params=None, device="cuda", seed=0, direct=()):
+    def __init__(self, params=None, device="cuda", seed=0, direct=(), fp32_direct=()):
         import torch
         p = init_params(seed) if params is None else np.asarray(params, np.float32)
         if p.shape != (PARAMS,):
             raise ValueError(f"router parameters must have {PARAMS} entries")
         self.params = torch.from_numpy(p.copy()).to(device)
-        bad = set(direct) - set(BUCKETS)
+        bad = (set(direct) | set(fp32_direct)) - set(BUCKETS)
         if bad:
             raise ValueError(f"direct buckets must be among {BUCKETS}, got {sorted(bad)}")
+        if set(direct) & set(fp32_direct):
+            raise ValueError("a bucket is either float64 direct or fp32 direct")
         self.direct = frozenset(direct)  # size buckets solved in float64 without the network
+        self.fp32_direct = frozenset(fp32_direct)  # size buckets solved by fp32 + refinement without it
+        self.policy_times = {}  # {size: {"routed", "fp64", "fp32": ns per matrix}} behind the choice, as loaded
 
     # ---------------------------------------------------------------- inference
     def forward(self, feat):
@@ -346,7 +362,7 @@ class Router:
         import json
         import pathlib
         np.save(path, self.params.cpu().numpy())
-        meta = {"direct_buckets": sorted(self.direct)}
+        meta = {"direct_buckets": sorted(self.direct), "fp32_direct_buckets": sorted(self.fp32_direct)}
         if report is not None:
             meta["training"] = report
         pathlib.Path(path).with_suffix(".json").write_text(json.dumps(meta, indent=1, default=str) + "\n")
@@ -356,15 +372,21 @@ class Router:
         import json
         import pathlib
         meta = pathlib.Path(path).with_suffix(".json")
-        direct = json.loads(meta.read_text()).get("direct_buckets", []) if meta.exists() else []
-        return cls(np.load(path, allow_pickle=False), device=device, direct=direct)
+        m = json.loads(meta.read_text()) if meta.exists() else {}
+        r = cls(np.load(path, allow_pickle=False), device=device, direct=m.get("direct_buckets", []),
+                fp32_direct=m.get("fp32_direct_buckets", []))
+        r.policy_times = m.get("training", {}).get("policy_ns_per_matrix", {})
+        return r
 
     # -------------------------------------------------------------- routed posv
     def posv(self, A, B, X, max_iter=10, tol=1e-14, info=None, stat=None, tier=None, feat=None, work=None):
         """Routed posv (float64): X := A^-1 B; stat / tier / feat as documented
         in include/tsdx_hip.h (tsdx_hip_posv_routed_f64).  For a size in
         self.direct every matrix takes the float64 tier without the network
-        (tier 2, stat -1, feat left untouched)."""
+        (tier 2, stat -1, feat left untouched); for a size in self.fp32_direct
+        every matrix takes the fp32 refinement tier without it (tier 1, stat
+        as tsdx_hip_posv_ir_f64: corrections used, -1 where the float64
+        fallback solved the matrix; feat left untouched)."""
         import torch
         c = Call(A, dtypes=(torch.float64,))
         batch, n = c.batch, c.n
@@ -386,6 +408,14 @@ class Router:
             if tier is not None:
                 tier.fill_(2)
             return X, info, stat, tier
+        if bucket(n) in self.fp32_direct:
+            from . import device
+            wi = work if work is not None and work.numel() >= _lib.lib().tsdx_hip_posv_ir_workspace_size(batch) \
+                else None
+            device.posv_ir(A, B, X, factor="fp32", max_iter=max_iter, tol=tol, info=info, stat=stat, work=wi)
+            if tier is not None:
+                tier.fill_(1)
+            return X, info, stat, tier
         need = _lib.lib().tsdx_hip_posv_routed_workspace_size(batch)
         if work is None:
             work = torch.empty(need, dtype=torch.uint8, device=A.device)
@@ -400,8 +430,19 @@ if __name__ == "__main__":
     import json
     ap = argparse.ArgumentParser(description="train the default precision router on the GPU")
     ap.add_argument("--train", action="store_true")
+    ap.add_argument("--choose-direct", action="store_true",
+                    help="re-time the per-size policies of the shipped router (parameters unchanged)")
     ap.add_argument("--out", default=str(DEFAULT_PARAMS))
     ap.add_argument("--epochs", type=int, default=60)
     args = ap.parse_args()
     if args.train:
         print(json.dumps(train_default(args.out, epochs=args.epochs)), flush=True)
+    elif args.choose_direct:
+        import pathlib
+        r = default()
+        report = json.loads(DEFAULT_PARAMS.with_suffix(".json").read_text()).get("training", {})
+        for k in ("direct_buckets", "fp32_direct_buckets", "routed_vs_fp64_ns_per_matrix", "policy_ns_per_matrix"):
+            report.pop(k, None)
+        report.update(direct_policy(r))
+        r.save(pathlib.Path(args.out), report)
+        print(json.dumps(report), flush=True)
