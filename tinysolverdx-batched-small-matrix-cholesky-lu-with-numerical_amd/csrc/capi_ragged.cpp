@@ -33,7 +33,7 @@ namespace {
 
 constexpr int kClasses = 3;
 
-int nvariants(bool dbl, int cls) { return cls == 0 ? 3 : (cls == 1 ? 2 : (dbl ? 1 : 3)); }
+int nvariants(bool dbl, int cls) { return cls == 0 ? 3 : (cls == 1 ? 2 : (dbl ? 2 : 3)); }
 
 // Type-erased access to the compiled (class, variant) launchers.
 template <class T>
@@ -59,7 +59,7 @@ template <>
 VariantOps<double> variant_ops<double>(int cls, int var) {
   if (cls == 0) return var == 0 ? ops_of<double, 0, 0>() : var == 1 ? ops_of<double, 0, 1>() : ops_of<double, 0, 2>();
   if (cls == 1) return var == 0 ? ops_of<double, 1, 0>() : ops_of<double, 1, 1>();
-  return ops_of<double, 2, 0>();
+  return var == 0 ? ops_of<double, 2, 0>() : ops_of<double, 2, 3>();  // variant 3: 3 waves per SIMD
 }
 
 int bucket_code(int kind, int n) {

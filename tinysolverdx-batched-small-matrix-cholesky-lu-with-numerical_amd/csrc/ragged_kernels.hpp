@@ -63,21 +63,25 @@ constexpr int rbucket_nb(int code) {
 
 // Compiled variants per (type, class): waves per workgroup and rows per lane of
 // the Cholesky buckets (matrices per wave = 64 / (NB / R)), and of the LU
-// bucket n <= 64 (RL64).  The autotuner times the variants of each class on
+// bucket n <= 64 (RL64); OCC, waves per SIMD the register allocation must
+// allow (1: no request).  The autotuner times the variants of each class on
 // the plan's size mix and keeps the best.
 template <class T, int CLS, int VAR>
 struct RaggedVariant;
 // class 0 (n <= 16)
-template <class T> struct RaggedVariant<T, 0, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1; };
-template <class T> struct RaggedVariant<T, 0, 1> { static constexpr int WPB = 4, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1; };
-template <class T> struct RaggedVariant<T, 0, 2> { static constexpr int WPB = 1, R4 = 1, R8 = 2, R16 = 4, R32 = 1, R64 = 1, RL64 = 1; };
+template <class T> struct RaggedVariant<T, 0, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1, OCC = 1; };
+template <class T> struct RaggedVariant<T, 0, 1> { static constexpr int WPB = 4, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1, OCC = 1; };
+template <class T> struct RaggedVariant<T, 0, 2> { static constexpr int WPB = 1, R4 = 1, R8 = 2, R16 = 4, R32 = 1, R64 = 1, RL64 = 1, OCC = 1; };
 // class 1 (n <= 32)
-template <class T> struct RaggedVariant<T, 1, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1; };
-template <class T> struct RaggedVariant<T, 1, 1> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 4, R64 = 1, RL64 = 1; };
+template <class T> struct RaggedVariant<T, 1, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1, OCC = 1; };
+template <class T> struct RaggedVariant<T, 1, 1> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 4, R64 = 1, RL64 = 1, OCC = 1; };
 // class 2 (n <= 64); two rows per lane only for float (registers)
-template <class T> struct RaggedVariant<T, 2, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1; };
-template <class T> struct RaggedVariant<T, 2, 1> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 2, RL64 = 1; };
-template <class T> struct RaggedVariant<T, 2, 2> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 2, RL64 = 2; };
+template <class T> struct RaggedVariant<T, 2, 0> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1, OCC = 1; };
+template <class T> struct RaggedVariant<T, 2, 1> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 2, RL64 = 1, OCC = 1; };
+template <class T> struct RaggedVariant<T, 2, 2> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 2, RL64 = 2, OCC = 1; };
+// float64: variant 0 asked to fit 3 waves per SIMD (168 VGPRs; the Cholesky
+// path alone needs 158, the LU path more)
+template <class T> struct RaggedVariant<T, 2, 3> { static constexpr int WPB = 1, R4 = 1, R8 = 1, R16 = 1, R32 = 1, R64 = 1, RL64 = 1, OCC = 3; };
 
 // Matrices per wave task for a bucket under a variant.
 template <class V>
@@ -181,7 +185,8 @@ constexpr int rvariant_lds_rows() {  // per-wave LDS in elements / (2 * 64)
 // Persistent kernel of one register class: each wave takes tasks from the
 // class counter until the list is exhausted (every wave reaches the exit).
 template <class T, int CLS, int VAR>
-__global__ __launch_bounds__((kWave * RaggedVariant<T, CLS, VAR>::WPB)) void k_ragged(RaggedArgs<T> a,
+__global__ __launch_bounds__((kWave * RaggedVariant<T, CLS, VAR>::WPB))
+__attribute__((amdgpu_waves_per_eu(RaggedVariant<T, CLS, VAR>::OCC))) void k_ragged(RaggedArgs<T> a,
                                                                                  const RTask* __restrict__ tasks,
                                                                                  int ntasks, int* __restrict__ counter) {
   using V = RaggedVariant<T, CLS, VAR>;
