@@ -3,7 +3,7 @@ the float64 CPU oracle (oracle/mlp_oracle.py, itself pinned by finite
 differences), training, and the routed posv.
 
 The gain test times the shipped router's real calls (features and network
-included) against the fixed tiers and the size rule alone on fresh matrices.
+included) against the fixed tiers and the network alone on fresh matrices.
 """
 import numpy as np
 import pytest
@@ -180,12 +180,14 @@ def test_router_gain_measured(tsdx):
     """What the shipped router gains, timed on the real calls (features and
     network included), on fresh matrices built like the routed-posv workload
     (kappa log-uniform [1, 1e12]; 16384 per size).  Policies per size:
-      routed      Router.posv as shipped (size rule, then the network),
+      routed      Router.posv as shipped (its per-size policy: the network,
+                  or one tier for every matrix where timing found that
+                  faster; shipped: float64 at n <= 32, fp32 + refinement at 64),
       network     Router.posv with the network at every size,
       fp32        fp32 factor + refinement for every matrix, float64 fallback,
       fp64        the float64 tier for every matrix,
-      rule        the size rule without the network (float64 for the direct
-                  sizes, fp32 + refinement with fallback otherwise).
+      rule        float64 for the float64-direct sizes, fp32 + refinement with
+                  fallback otherwise, without the network.
     Asserted: every routed answer meets 1e-14, and over the mix of sizes the
     routed policy beats both fixed tiers.  Measured, not asserted as a gain:
     at n = 64 the network against fp32 for all (it does not win there: the
