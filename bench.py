@@ -367,13 +367,7 @@ def run_posv(a):
                                                  for r in (routed_rate, mixed_rate, plain_rate)]},
         "roofline": roofline,
     }
-    if a.copy_probe:
-        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log("[bench] copy probe: 3 x 1 GiB device copies done")
+    _copy_probe(a)
     if not a.no_cpu:
         log("[bench] CPU baseline (reference posv on the host cores)")
         out["cpu_baseline"] = cpu_baseline(A, B, nrhs)
@@ -389,6 +383,20 @@ def run_posv(a):
 # restores A from a pristine copy and pushes A out of the 256 MB Infinity Cache
 # (an untimed read-modify-write of 1 GiB); the step's time is the HIP-event
 # time of the factorization alone, summed over the K steps.
+
+def _copy_probe(a):
+    """With --copy-probe: three plain 1 GiB device copies, the reference
+    kernel for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes."""
+    if not a.copy_probe:
+        return
+    import torch
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("[bench] copy probe: 3 x 1 GiB device copies done")
+
 
 def _inplace_steps(fn, restore, steps, warmup):
     import torch
@@ -624,6 +632,7 @@ def run_ragged(a):
     plan = device.RaggedPlan(A, d_off, d_size, d_kind, piv, d_poff, info, tune=True)
     log(f"[bench] plan + autotune {time.perf_counter() - t0:.1f}s: {plan.report}")
     wall, ev = _inplace_steps(plan.run, lambda: A.copy_(A0), a.steps, a.warmup)
+    _copy_probe(a)
     ms = ev / a.steps * 1e3
     es = 4 if a.dtype == "f32" else 8
     n64 = sizes.astype(np.int64)
