@@ -721,7 +721,8 @@ def run_ragged(a):
            "wall_ms_per_step_incl_restore": round(wall / a.steps * 1e3, 4)}
     return _line(f"ragged factor matrices/s {a.dtype} (1M, n 2..64, half Cholesky half LU); HBM GB/s",
                  count / (ms * 1e-3), "matrices/s", a, ms, a.dtype, cfg,
-                 _roof(total, ms * 1e-3, "k_ragged<T,class,variant> x3 (one graph)", round(total / count, 1)), cpu)
+                 _roof(total, ms * 1e-3, "k_ragged<T,class,variant> x3 (one graph)", round(total / count, 1),
+                       f"ragged_{a.dtype}_{count}"), cpu)
 
 
 
