@@ -533,6 +533,7 @@ def run_getrf(a):
     piv = torch.zeros((batch, n), dtype=torch.int32, device="cuda")
     info = torch.zeros(batch, dtype=torch.int32, device="cuda")
     wall, ev = _inplace_steps(lambda: device.getrf(A, piv, info), lambda: A.copy_(A0), a.steps, a.warmup)
+    _copy_probe(a)
     ms = ev / a.steps * 1e3
     bad = int((info != 0).sum().item())
     B = torch.randn((batch, n, 4), generator=g, device="cuda", dtype=dt)
