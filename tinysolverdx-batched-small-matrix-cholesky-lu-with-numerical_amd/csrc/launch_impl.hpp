@@ -38,11 +38,14 @@ hipError_t launch_potrf(T* A, int n, int64_t batch, int* info, int vec, hipStrea
 // at 16384, 1175 vs 1291 us at 65536; both give identical results).
 constexpr int64_t kGetrfRowsMinBatch = 16384;
 
-template <class T, int NB, bool Z>
-static void getrf_go(T* A, int n, int64_t batch, int* piv, int* info, int vec, hipStream_t s) {
+// In the n <= 64 bucket, sizes up to 48 take the kernels that skip the columns
+// past n, 16 at a time (COLS); at 49..64 every run of 16 columns holds data.
+template <class T, int NB, bool Z, bool COLS>
+static void getrf_go_cols(T* A, int n, int64_t batch, int* piv, int* info, int vec, hipStream_t s) {
   using S = GetrfShape<T, NB>;
   auto one_row = [&] {
-    hipLaunchKernelGGL((k_getrf<T, NB, Z>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info, vec);
+    hipLaunchKernelGGL((k_getrf<T, NB, Z, COLS>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, piv, info,
+                       vec);
   };
   if constexpr (S::R == 1) {
     one_row();
@@ -53,9 +56,19 @@ static void getrf_go(T* A, int n, int64_t batch, int* piv, int* info, int vec, h
     }
     constexpr int64_t per_block = (int64_t)S::WPB * (kWave / S::L);
     const dim3 grid((unsigned)((batch + per_block - 1) / per_block));
-    hipLaunchKernelGGL((k_getrf_rows<T, S::L, S::R, S::WPB, Z>), grid, dim3(kWave * S::WPB), 0, s, A, n, batch, piv,
-                       info, vec);
+    hipLaunchKernelGGL((k_getrf_rows<T, S::L, S::R, S::WPB, Z, COLS>), grid, dim3(kWave * S::WPB), 0, s, A, n, batch,
+                       piv, info, vec);
   }
+}
+template <class T, int NB, bool Z>
+static void getrf_go(T* A, int n, int64_t batch, int* piv, int* info, int vec, hipStream_t s) {
+  if constexpr (NB == 64) {
+    if (n <= 48) {
+      getrf_go_cols<T, NB, Z, true>(A, n, batch, piv, info, vec, s);
+      return;
+    }
+  }
+  getrf_go_cols<T, NB, Z, false>(A, n, batch, piv, info, vec, s);
 }
 
 template <class T, int NB>

@@ -370,7 +370,9 @@ constexpr int getrf_occupancy() {
   return NB == 32 ? 5 : 1;
 }
 
-template <class T, int NB, bool ZTILE>
+// COLS: the columns at or past n (rounded up to 16) are skipped (lu_regs),
+// for n well below the bucket's width.
+template <class T, int NB, bool ZTILE, bool COLS = false>
 __global__ __launch_bounds__(kWave * waves_per_block<NB>()) __attribute__((amdgpu_waves_per_eu(getrf_occupancy<T, NB>()))) void k_getrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ piv,
                                                  int* __restrict__ info, int vec) {
   __shared__ __align__(16) T rowb_s[waves_per_block<NB>()][2 * kWave];
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) __attribute__((amdgp
   T r[NB];
   load_row<T, NB>(r, base + (int64_t)t.i * n, n, io, vec != 0);
   int pos, mypiv, inf;
-  lu_regs<T, NB, ZTILE>(r, n, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
+  lu_regs<T, NB, ZTILE, COLS>(r, n, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
   if constexpr (lu_stage_rounds<T, NB>() > 0) {
     if (vec && n == NB) {  // uniform
       store_rows_lines<T, NB>(r, t.g, pos, io && (unsigned)pos < (unsigned)NB, t.lane, stage_s[wv], A, t.m - t.g,

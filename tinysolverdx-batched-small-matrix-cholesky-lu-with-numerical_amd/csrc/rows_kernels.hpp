@@ -379,7 +379,7 @@ struct GetrfShape<float, 64> {
   static constexpr int L = 32, R = 2, WPB = 1;
 };
 
-template <class T, int L, int R, int WPB, bool ZTILE>
+template <class T, int L, int R, int WPB, bool ZTILE, bool COLS = false>
 __global__ __launch_bounds__(kWave * WPB) void k_getrf_rows(T* __restrict__ A, int n, int64_t batch,
                                                            int* __restrict__ piv, int* __restrict__ info, int vec) {
   constexpr int N = L * R;
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(kWave * WPB) void k_getrf_rows(T* __restrict__ A, i
     load_row<T, N>(r[s], A + m * nn + (int64_t)row * n, n, live && row < n, vec != 0);
   });
   int pos[R], mypiv[R], inf;
-  lu_rows<T, L, R, ZTILE>(r, n, n, q, g, rowb_s[wv], pos, mypiv, inf);
+  lu_rows<T, L, R, ZTILE, COLS>(r, n, n, q, g, rowb_s[wv], pos, mypiv, inf);
   static_for<0, R>([&](auto s) {
     const int row = q + L * s;
     if (live && row < n) {
