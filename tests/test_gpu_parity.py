@@ -121,7 +121,7 @@ def test_failed_cholesky_matches_tile_semantics(hip, oracle, dt):
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_fused_drivers_vs_oracle(hip, oracle, dt):
     rng = np.random.default_rng(9)
-    for n in (1, 2, 3, 7, 8, 12, 16, 17, 24, 25, 32, 40, 63, 64):
+    for n in (1, 2, 3, 7, 8, 12, 16, 17, 24, 25, 32, 33, 40, 48, 49, 63, 64):
         for nrhs in (1, 3, 4, 9):
             print(f"[fused] {np.dtype(dt).name} n={n} nrhs={nrhs}", flush=True)
             S = _spd(rng, 23, n, dt)

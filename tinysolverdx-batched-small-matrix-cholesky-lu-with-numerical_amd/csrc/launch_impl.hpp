@@ -122,9 +122,13 @@ static void gesv_go(const T* A, T* LUout, int n, int64_t batch, const T* B, T* X
   if constexpr (NB <= 16) {
     hipLaunchKernelGGL((k_gesv<T, NB, NR, true>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, LUout, n, batch, B, X,
                        nrhs, piv, info, vec);
-  } else if constexpr (NB == 64) {
-    hipLaunchKernelGGL((k_gesv<T, NB, NR, false>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, LUout, n, batch, B, X,
-                       nrhs, piv, info, vec);
+  } else if constexpr (NB == 64) {  // n <= 48: the LU skips the columns past n (as getrf_go)
+    if (n <= 48)
+      hipLaunchKernelGGL((k_gesv<T, NB, NR, false, true>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, LUout, n,
+                         batch, B, X, nrhs, piv, info, vec);
+    else
+      hipLaunchKernelGGL((k_gesv<T, NB, NR, false>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, LUout, n, batch, B,
+                         X, nrhs, piv, info, vec);
   } else {
     if (ztile)
       hipLaunchKernelGGL((k_gesv<T, NB, NR, true>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, LUout, n, batch, B, X,

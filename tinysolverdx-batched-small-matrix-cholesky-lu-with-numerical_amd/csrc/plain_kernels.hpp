@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_posv(const T*
 }
 
 // gesv: getrf + getrs with the factor kept in registers.
-template <class T, int NB, int NR, bool ZTILE>
+template <class T, int NB, int NR, bool ZTILE, bool COLS = false>
 __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_gesv(const T* __restrict__ A, T* __restrict__ LUout, int n, int64_t batch,
                                                 const T* __restrict__ Bin, T* __restrict__ X, int nrhs,
                                                 int* __restrict__ piv, int* __restrict__ info, int vec) {
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_gesv(const T*
   T r[NB];
   load_row<T, NB, false>(r, A + t.m * nn + (int64_t)t.i * n, n, io, vec != 0);
   int pos, mypiv, inf;
-  lu_regs<T, NB, ZTILE>(r, n, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
+  lu_regs<T, NB, ZTILE, COLS>(r, n, n, t.i, t.g, rowb_s[wv], pos, mypiv, inf);
   if (io) {
     if (LUout) store_row<T, NB, false>(r, LUout + t.m * nn + (int64_t)pos * n, n, vec != 0, false, 0);
     if (piv) piv[t.m * n + t.i] = mypiv;
