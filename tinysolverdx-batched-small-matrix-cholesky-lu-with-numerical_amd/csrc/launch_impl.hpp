@@ -18,16 +18,31 @@ inline dim3 block_for() {
   return dim3(kWave * waves_per_block<NB>());
 }
 
-template <class T, int NB>
-hipError_t launch_potrf(T* A, int n, int64_t batch, int* info, int vec, hipStream_t s) {
+template <class T, int NB, bool COLS>
+static void potrf_go(T* A, int n, int64_t batch, int* info, int vec, hipStream_t s) {
   using S = PotrfShape<T, NB>;
   if constexpr (S::R == 1) {
-    hipLaunchKernelGGL((k_potrf<T, NB>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, info, vec);
+    hipLaunchKernelGGL((k_potrf<T, NB, COLS>), grid_for<NB>(batch), block_for<NB>(), 0, s, A, n, batch, info, vec);
   } else {
     constexpr int64_t per_block = (int64_t)S::WPB * (kWave / S::L);
     const dim3 grid((unsigned)((batch + per_block - 1) / per_block));
-    hipLaunchKernelGGL((k_potrf_rows<T, S::L, S::R, S::WPB, S::OCC, S::LOWER, S::STAGE, S::CHUNK>), grid, dim3(kWave * S::WPB), 0, s, A, n, batch, info, vec);
+    hipLaunchKernelGGL((k_potrf_rows<T, S::L, S::R, S::WPB, S::OCC, S::LOWER, S::STAGE, S::CHUNK, COLS>), grid,
+                       dim3(kWave * S::WPB), 0, s, A, n, batch, info, vec);
   }
+}
+// In the n <= 64 bucket, the one-row kernel skips the columns past n, 16 at a
+// time, for sizes up to 48 (COLS, as the LU launchers below; fp64 n=40 0.465
+// -> 0.435 ms at 512 MiB of A).  The multi-row kernel gained 2-8 % at n=40..48
+// but ran slower at n=33 with it, so it keeps every column.
+template <class T, int NB>
+hipError_t launch_potrf(T* A, int n, int64_t batch, int* info, int vec, hipStream_t s) {
+  if constexpr (NB == 64 && PotrfShape<T, NB>::R == 1) {
+    if (n <= 48) {
+      potrf_go<T, NB, true>(A, n, batch, info, vec, s);
+      return hipGetLastError();
+    }
+  }
+  potrf_go<T, NB, false>(A, n, batch, info, vec, s);
   return hipGetLastError();
 }
 

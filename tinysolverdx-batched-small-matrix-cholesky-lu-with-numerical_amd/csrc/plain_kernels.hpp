@@ -40,9 +40,12 @@ namespace tsdx_hip {
 // independent updates of step k instead of following them.
 // nloop: wave-uniform step bound (the largest size among the wave's groups);
 // n: this group's size (info only counts steps k < n; steps past a group's
-// size touch only its zero padding).
-template <class T, int NB>
+// size touch only its zero padding).  COLS: columns at or past nloop (rounded
+// up to 16) are not updated; they lie right of every row below nloop, which is
+// all the factor a caller stores.
+template <class T, int NB, bool COLS = false>
 __device__ __forceinline__ int chol_regs(T (&r)[NB], int nloop, int n, int i, int g, T* col) {
+  const int bound = COLS ? __builtin_amdgcn_readfirstlane(nloop) : NB;
   int info = 0;
   auto pivot = [&](auto kc) {
     constexpr int k = decltype(kc)::value;
@@ -62,7 +65,7 @@ __device__ __forceinline__ int chol_regs(T (&r)[NB], int nloop, int n, int i, in
       wave_sync();
       r[k + 1] = fmaT(-r[k], cb[k + 1], r[k + 1]);
       pivot(ic<k + 1>{});
-      static_for<k + 2, NB>([&](auto jc) {
+      static_for_cols<k + 2, NB, COLS>(bound, [&](auto jc) {
         constexpr int j = decltype(jc)::value;
         r[j] = fmaT(-r[k], cb[j], r[j]);
       });
@@ -336,7 +339,7 @@ __device__ __forceinline__ void store_rows_lines(const T (&r)[NB], int g, int po
   });
 }
 
-template <class T, int NB>
+template <class T, int NB, bool COLS = false>
 __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_potrf(T* __restrict__ A, int n, int64_t batch, int* __restrict__ info, int vec) {
   __shared__ T col_s[waves_per_block<NB>()][2 * kWave];
   __shared__ __align__(16) T stage_s[waves_per_block<NB>()][lu_stage_elems<T, NB>()];
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(kWave * waves_per_block<NB>()) void k_potrf(T* __re
   const bool io = t.live && t.i < n;
   T r[NB];
   load_row<T, NB>(r, row, n, io, vec != 0);
-  const int inf = chol_regs<T, NB>(r, n, n, t.i, t.g, col_s[wv]);
+  const int inf = chol_regs<T, NB, COLS>(r, n, n, t.i, t.g, col_s[wv]);
   if constexpr (lu_stage_rounds<T, NB>() > 0) {
     if (vec && n == NB) {  // uniform: line-ordered stores as for the LU factor
       static_for<0, NB>([&](auto j) { r[j] = j > t.i ? T(0) : r[j]; });

@@ -315,7 +315,7 @@ __device__ __forceinline__ void store_slot_lines(const T (&w)[N], int row, int g
   });
 }
 
-template <class T, int L, int R, int WPB, int OCC, bool LOWER, bool STAGE, bool CHUNK>
+template <class T, int L, int R, int WPB, int OCC, bool LOWER, bool STAGE, bool CHUNK, bool COLS = false>
 __global__ __launch_bounds__(kWave * WPB) __attribute__((amdgpu_waves_per_eu(OCC))) void k_potrf_rows(
     T* __restrict__ A, int n, int64_t batch, int* __restrict__ info, int vec) {
   constexpr int N = L * R;
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kWave * WPB) __attribute__((amdgpu_waves_per_eu(OCC
     }
     if (live && row < n) store_row<T, N, CHUNK>(w, A + m * nn + (int64_t)row * n, n, vec != 0, true, row);
   };
-  const int inf = chol_rows<T, L, R>(r, n, n, q, g, col_s[wv], store_slot);
+  const int inf = chol_rows<T, L, R, decltype(store_slot), COLS>(r, n, n, q, g, col_s[wv], store_slot);
   store_slot(ic<R - 1>{});
   if (live && q == 0 && info) info[m] = inf;
 }
